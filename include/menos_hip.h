@@ -1,0 +1,254 @@
+/* menos_hip.h -- the C ABI of the MI355X dense statevector engine.
+ *
+ * A drop-in for MENOS's CUDA dense backend: the same engine contract
+ * (execution plan in; statevector, seeded counts and raw Pauli expectation
+ * terms out; device peak reported to the memory ledger), re-implemented as
+ * hand-written HIP kernels for gfx950 behind plain C entry points. No entry
+ * point throws; each returns an mh_status and, on failure, leaves a message
+ * for mh_last_error() on the calling thread. The C++ side maps statuses to
+ * the reference's exception types: MH_ERR_RESOURCE -> ResourceExceeded,
+ * MH_ERR_INTERNAL / MH_ERR_DEVICE -> InternalError (reference
+ * src/kernels/cuda/cuda_error.h:11-29, device_pool.cpp:68-92).
+ *
+ * Conventions (reference src/kernels/dense_kernels.h:25-47): qubits are
+ * little-endian (bit j of an amplitude index is qubit j); bit j of a gate
+ * matrix's row/column index is operand j in the operand list's own order;
+ * matrices are row-major 2^k x 2^k complex values given as interleaved
+ * (re, im) doubles; a state is 2^n contiguous amplitudes, interleaved
+ * (re, im), complex128 (MH_C128) or complex64 (MH_C64).
+ *
+ * The MH_C128 form is bitwise the reference CPU generic kernel
+ * (dense_kernels.cpp:465-524) on the same plan, and its counts are the
+ * reference sampler's (sample.cpp:24-75) for the same seed. The MH_C64 form
+ * stores complex64 amplitudes, computes gates in FP32 and keeps norms,
+ * expectations and probability prefix sums in FP64.
+ *
+ * Every call that touches device memory synchronises its stream before it
+ * returns (reference dense_kernels.h:204-206), so a following call sees its
+ * writes. Device pointers passed in are never freed by the callee.
+ */
+#ifndef MENOS_HIP_H
+#define MENOS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum mh_status {
+    MH_OK = 0,
+    MH_ERR_INVALID = 1,  /* a precondition of the call does not hold        */
+    MH_ERR_RESOURCE = 2, /* over budget or out of device memory (bytes named) */
+    MH_ERR_INTERNAL = 3, /* a HIP call failed                                */
+    MH_ERR_DEVICE = 4    /* no usable gfx950 device                          */
+} mh_status;
+
+typedef enum mh_precision { MH_C128 = 0, MH_C64 = 1 } mh_precision;
+
+/* The message of the calling thread's last failed call ("" if none). */
+const char* mh_last_error(void);
+
+/* ---- Device query. Replaces kernels::cuda device_info
+ * (reference src/kernels/cuda/device_info.h:14-55). -------------------------- */
+typedef struct mh_device_info {
+    char name[128];
+    char arch[64];             /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+    int32_t compute_units;
+    int32_t warp_size;
+    int32_t max_threads_per_block;
+    int32_t lds_bytes_per_block;
+    int32_t clock_khz;
+    int32_t memory_clock_khz;
+    int32_t memory_bus_width;
+    uint64_t total_bytes;
+    uint64_t free_bytes;
+} mh_device_info;
+
+int mh_device_query(mh_device_info* out);
+/* Free device bytes plus what the pool holds for reuse
+ * (reference device_info.h: device_available_bytes). */
+int mh_device_available_bytes(uint64_t* out);
+
+/* ---- Memory pool. Replaces DevicePool (reference device_pool.h:34-78): freed
+ * blocks are retained and reused, sized for 288 GB of HBM3E. --------------- */
+int mh_pool_alloc(uint64_t bytes, void** out);
+int mh_pool_free(void* ptr);
+int mh_pool_reusable_bytes(uint64_t* out);
+int mh_pool_trim(void);
+
+/* ---- State primitives. Replace kernels::cuda::{initialize, apply_matrix,
+ * apply_matrix_generic, expectation_of_term} (reference dense_kernels.h:198-258). */
+
+/* state := phase * |0...0>; the phase exp(i*global_phase) is computed by the
+ * caller on the host (reference dense_kernels.cpp:21-27). */
+int mh_state_init(void* state, uint32_t num_qubits, int32_t precision, double phase_re, double phase_im);
+
+/* One gate: a 2^k x 2^k matrix (host memory, 2*4^k doubles) at `operands`
+ * (host memory), k <= 6. Diagonal matrices take the streaming diagonal kernel,
+ * k = 1 and 2 their own kernels; every path is bitwise the generic kernel. */
+int mh_apply_matrix(void* state, uint32_t num_qubits, int32_t precision, const double* matrix,
+                    const uint32_t* operands, uint32_t k);
+
+/* The generic kernel alone, whatever the matrix (the oracle of the others,
+ * reference dense_kernels.h:227-235). */
+int mh_apply_matrix_generic(void* state, uint32_t num_qubits, int32_t precision, const double* matrix,
+                            const uint32_t* operands, uint32_t k);
+
+/* A whole block list in one call: block b applies the matrix at
+ * matrices + matrix_offsets[b] (in complex entries) to operands
+ * operands + operand_offsets[b] .. operand_offsets[b+1]. Blocks are grouped into
+ * LDS-window passes (many blocks per HBM read+write of the state); results are
+ * bitwise those of calling mh_apply_matrix block by block. window_qubits = 0
+ * picks the default window. On return *passes (if non-null) holds the number
+ * of HBM passes made. */
+int mh_apply_blocks(void* state, uint32_t num_qubits, int32_t precision, uint32_t num_blocks,
+                    const uint32_t* operand_offsets, const uint32_t* operands, const uint64_t* matrix_offsets,
+                    const double* matrices, uint32_t window_qubits, uint32_t* passes);
+
+/* Raw expectation terms <psi|P_t|psi> (no coefficient), one per (x, z) mask
+ * pair, out = 2 doubles per term. FP64 fixed-order reduction: reproducible
+ * run to run; equal to the reference's sequential sum within rounding
+ * (reference dense_kernels.cpp:544-569, CUDA contract dense_kernels.h:237-246). */
+int mh_expectation(const void* state, uint32_t num_qubits, int32_t precision, const uint64_t* x_masks,
+                   const uint64_t* z_masks, uint32_t num_terms, double* out);
+
+/* The basis index every sorted uniform lands on under the reference's
+ * sequential cumulative walk (reference sample.cpp:55-69): the first basis
+ * whose running FP64 sum of |a|^2 exceeds u, or the last basis if none does.
+ * Exact (bitwise the sequential sums) by an integer scan per binade.
+ * sorted_uniforms: host, ascending; basis_out: host, `shots` entries. */
+int mh_sample(const void* state, uint32_t num_qubits, int32_t precision, const double* sorted_uniforms,
+              uint64_t shots, uint64_t* basis_out);
+
+/* The first basis whose cumulative sum exceeds each uniform, for a
+ * non-sole region's sampler (reference cumulative_distribution.cpp:83-92);
+ * uniforms need not be sorted. */
+int mh_sample_unsorted(const void* state, uint32_t num_qubits, int32_t precision, const double* uniforms,
+                       uint64_t count, uint64_t* basis_out);
+
+int mh_copy_to_host(const void* state, uint32_t num_qubits, int32_t precision, void* host);
+int mh_copy_from_host(void* state, uint32_t num_qubits, int32_t precision, const void* host);
+/* FNV-1a 64 over the state's bytes, computed on the host after a copy-back
+ * in chunks (the reference's digest, tests/memory/memory_fixtures.cpp:599-645). */
+int mh_state_digest(const void* state, uint32_t num_qubits, int32_t precision, uint64_t* out);
+
+/* ---- The engine. Replaces CudaDenseEngine::{estimate, execute}
+ * (reference src/engines/dense_cuda/cuda_dense_engine.cpp:182-412) behind the
+ * SimulationEngine contract (include/menos/simulation_engine.h:322-367). ---- */
+
+/* A circuit in the reference's flat batch layout (Circuit::append_batch,
+ * src/ir/circuit.cpp:34-67). Opcodes are the reference's enum Opcode
+ * (include/menos/circuit.h); a Unitary op (21) carries its row-major matrix
+ * as 2*4^k doubles in params; a Measure op (22) carries (qubit, clbit) as its
+ * two qubits entries. */
+typedef struct mh_circuit {
+    uint32_t num_qubits;
+    uint32_t num_ops;
+    const int32_t* opcodes;
+    const uint32_t* qubit_offsets; /* num_ops + 1 */
+    const uint32_t* qubits;
+    const uint32_t* param_offsets; /* num_ops + 1 */
+    const double* params;
+} mh_circuit;
+
+/* How the gate list becomes blocks. MH_FUSE_NONE: one block per gate.
+ * MH_FUSE_CONTAINMENT: the reference's rule (src/ir/fuse_gates.cpp:190-283),
+ * its default. MH_FUSE_WIDTH: this engine's width-capped fusion (fusion_width
+ * operands at most, 2..6), order-preserving; the oracle applies the same
+ * block list. */
+typedef enum mh_fusion { MH_FUSE_NONE = 0, MH_FUSE_CONTAINMENT = 1, MH_FUSE_WIDTH = 2 } mh_fusion;
+
+typedef struct mh_execution_request {
+    const mh_circuit* circuit;
+    int32_t precision;
+    int32_t fusion;          /* mh_fusion */
+    uint32_t fusion_width;   /* MH_FUSE_WIDTH only */
+    uint32_t window_qubits;  /* 0: default */
+    double global_phase;
+    uint64_t shots;          /* 0: no counts */
+    uint64_t seed;
+    const uint32_t* target_qubits; /* measure targets (qubit -> clbit) */
+    const uint32_t* target_clbits;
+    uint32_t num_targets;
+    uint32_t bitstring_width;      /* <= 64 in this ABI */
+    uint32_t num_terms;            /* 0: no expectation */
+    const uint64_t* x_masks;
+    const uint64_t* z_masks;
+    uint64_t budget_bytes;         /* device budget; 0 = no limit */
+    int32_t want_statevector;
+    int32_t record_pass_times;     /* per-pass device times into the result */
+} mh_execution_request;
+
+typedef struct mh_execution_result {
+    /* Caller-owned outputs, any may be null when not requested. */
+    void* statevector;             /* 2^n amplitudes of the request's precision */
+    uint64_t* outcomes;            /* up to `shots` distinct outcomes, ascending */
+    uint64_t* counts;
+    uint64_t num_outcomes;         /* written */
+    double* expectation_terms;     /* 2 doubles per term */
+    float* pass_ms;                /* record_pass_times: one per pass (capacity pass_ms_capacity) */
+    uint32_t pass_ms_capacity;
+    /* Filled by the engine. */
+    uint32_t num_blocks;
+    uint32_t num_passes;
+    uint64_t fused_blocks;
+    uint64_t gates_absorbed;
+    uint64_t actual_bytes;         /* device bytes of the state */
+    uint64_t peak_device_bytes;    /* the memory ledger's device peak for this call */
+    double execute_seconds;        /* host wall time of execute */
+    double gate_seconds;           /* device time of the gate passes */
+    double sample_seconds;
+    double expectation_seconds;
+} mh_execution_result;
+
+typedef struct mh_cost_estimate {
+    int32_t feasible;
+    char infeasible_reason[256];
+    uint64_t device_bytes;   /* state + scratch (reference plan.h DeviceCost) */
+    uint64_t host_bytes;
+    uint64_t work_units;     /* 2^n + sum over blocks 2^(n+k) + outputs (dense_engine.cpp:309-331) */
+    uint32_t blocks;
+    uint32_t passes;         /* HBM passes of the window schedule */
+    uint64_t launches;
+    uint64_t h2d_bytes;
+    uint64_t d2h_bytes;
+    double seconds;          /* the calibrated time model's prediction */
+} mh_cost_estimate;
+
+/* Never fails on an infeasible request: it reports feasible = 0 and a reason
+ * (reference simulation_engine.h:349-359). */
+int mh_engine_estimate(const mh_execution_request* request, mh_cost_estimate* out);
+int mh_engine_execute(const mh_execution_request* request, mh_execution_result* result);
+
+/* The fusion plan the engine would run (FusionPlan layout, reference
+ * src/ir/fuse_gates.h:38-76). Outputs sized for the worst case by the caller:
+ * order[num_ops], block_begin[num_ops+2], block_qubits[sum of operand counts],
+ * qubits_begin[num_ops+2]. */
+int mh_plan_fusion(const mh_circuit* circuit, int32_t fusion, uint32_t fusion_width, uint32_t* num_blocks,
+                   uint32_t* order, uint32_t* block_begin, uint32_t* block_qubits, uint32_t* qubits_begin,
+                   uint64_t* fused_blocks, uint64_t* gates_absorbed);
+
+/* A named gate's matrix as the engine generates it on the host (reference
+ * src/ir/gate_table.cpp:19-265); returns k through *k. */
+int mh_gate_matrix(int32_t opcode, const double* params, double* out, uint32_t* k);
+
+/* ---- Calibration for the time model (reference src/plan/cuda_calibration.cpp:42-153,
+ * HardwareProfile cuda_* fields, hardware_profile.h:52-77). ------------------ */
+typedef struct mh_calibration {
+    double hbm_bytes_per_second;     /* measured copy bandwidth (read + write) */
+    double pass_bytes_per_second;    /* a window pass at 24 qubits, c128 */
+    double launch_seconds;           /* an empty launch, synchronised */
+    double h2d_bytes_per_second;     /* pinned */
+    double d2h_bytes_per_second;     /* pinned */
+    double fp64_ops_per_second;      /* non-fused DMUL/DADD */
+} mh_calibration;
+
+int mh_calibrate(mh_calibration* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MENOS_HIP_H */

@@ -1,0 +1,99 @@
+// HipDenseEngine: the MI355X dense statevector engine behind MENOS's
+// SimulationEngine contract (reference include/menos/simulation_engine.h:322-367).
+//
+// It is the {Dense, Hip, InMemory} engine: estimate() never throws and reports
+// an infeasible request with a reason; execute() takes the region's
+// operations and output request and returns the statevector, counts (sole
+// region) and the raw, coefficient-free expectation terms, charging the
+// device bytes it holds to the calling thread's ledger. The request and result
+// shapes mirror EngineExecutionRequest / EngineExecutionResult
+// (simulation_engine.h); the C ABI (menos_hip.h) carries them across.
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <optional>
+#include <string>
+#include <vector>
+
+#include "mh_internal.h"
+
+namespace mh {
+
+// The calling thread's device ledger (reference src/common/memory_ledger.h:37-117,
+// Space::Device): live and peak bytes the engine holds on the device.
+class DeviceLedger {
+public:
+    static DeviceLedger& current();
+    void charge(uint64_t bytes);
+    void release(uint64_t bytes);
+    void reset();
+    uint64_t peak() const { return peak_; }
+    uint64_t live() const { return live_; }
+
+private:
+    uint64_t live_ = 0;
+    uint64_t peak_ = 0;
+};
+
+// A device allocation charged to the ledger for as long as it lives.
+class ChargedBuffer {
+public:
+    explicit ChargedBuffer(uint64_t bytes) : buf_(bytes), bytes_(bytes) { DeviceLedger::current().charge(bytes); }
+    ~ChargedBuffer() { DeviceLedger::current().release(bytes_); }
+    ChargedBuffer(const ChargedBuffer&) = delete;
+    ChargedBuffer& operator=(const ChargedBuffer&) = delete;
+    void* get() const { return buf_.get(); }
+
+private:
+    DeviceBuffer buf_;
+    uint64_t bytes_;
+};
+
+struct ExecutionRequest {
+    const mh_circuit* circuit = nullptr;
+    int32_t precision = MH_C128;
+    int32_t fusion = MH_FUSE_CONTAINMENT;
+    uint32_t fusion_width = 0;
+    uint32_t window_qubits = 0;
+    double global_phase = 0.0;
+    bool want_statevector = false;
+    uint64_t shots = 0;
+    uint64_t seed = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> measure_targets;
+    uint32_t bitstring_width = 0;
+    std::vector<std::pair<uint64_t, uint64_t>> terms;  // (x, z)
+    uint64_t budget_bytes = 0;
+    bool record_pass_times = false;
+};
+
+struct ExecutionResult {
+    std::optional<std::vector<unsigned char>> statevector;  // raw amplitudes of the request's precision
+    std::optional<std::map<uint64_t, uint64_t>> counts;
+    std::optional<std::vector<cplx>> expectation_terms;
+    uint64_t actual_bytes = 0;
+    uint64_t peak_device_bytes = 0;
+    uint32_t num_blocks = 0;
+    uint32_t num_passes = 0;
+    uint64_t fused_blocks = 0, gates_absorbed = 0;
+    std::vector<float> pass_ms;
+    double execute_seconds = 0, gate_seconds = 0, sample_seconds = 0, expectation_seconds = 0;
+};
+
+struct CostEstimate {
+    bool feasible = false;
+    std::string infeasible_reason;
+    uint64_t device_bytes = 0, host_bytes = 0, work_units = 0;
+    uint32_t blocks = 0, passes = 0;
+    uint64_t launches = 0, h2d_bytes = 0, d2h_bytes = 0;
+    double seconds = 0;
+};
+
+class HipDenseEngine {
+public:
+    const char* name() const { return "dense_hip"; }
+    CostEstimate estimate(const ExecutionRequest& request) const noexcept;
+    ExecutionResult execute(const ExecutionRequest& request) const;
+};
+
+}  // namespace mh

@@ -1,0 +1,225 @@
+// Pauli-string expectation reductions (FP64 for both precisions).
+//
+// The raw term of P = (x, z) is (reference src/kernels/dense_kernels.h:145-194,
+// dense_kernels.cpp:544-569)
+//
+//     i^popcount(x & z) * sum_m conj(psi[m]) * psi[m ^ x] * (-1)^popcount(z & (m ^ x)).
+//
+// Terms sharing an X mask share their products, so one launch reads the state
+// once for up to kTermsPerLaunch such terms. For x != 0 the sum is taken over
+// pairs (m, m' = m ^ x) with bit h = msb(x) of m clear: with p = conj(psi[m]) *
+// psi[m'], the pair contributes p * s(m') + conj(p) * s(m), s(k) =
+// (-1)^popcount(z & k); conj(p) is bitwise conj(psi[m']) * psi[m], so each
+// product is the reference's, read once. Each thread sums a fixed grid-stride
+// set of items in order, then the workgroup and the partials are added in a fixed
+// tree order: the value is reproducible run to run and differs from the
+// reference's sequential index-order sum by rounding only (the reference's
+// own CUDA contract, dense_kernels.h:237-246).
+#include <algorithm>
+#include <map>
+
+#include "mh_internal.h"
+
+namespace mh {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr uint32_t kTermsPerLaunch = 16;
+constexpr uint32_t kMaxBlocks = 2048;
+
+struct ExpArgs {
+    const void* state;
+    double* partials;   // [term][block] complex as 2 doubles
+    uint64_t pairs;     // work items: 2^n (x == 0) or 2^(n-1)
+    uint64_t x;
+    uint32_t h;         // msb of x (unused when x == 0)
+    uint32_t terms;
+    uint32_t blocks;
+    uint64_t z[kTermsPerLaunch];
+};
+
+template <typename V>
+__device__ __forceinline__ void load_amp(const V* s, uint64_t i, double& re, double& im) {
+    const V v = s[i];
+    re = static_cast<double>(v.x);
+    im = static_cast<double>(v.y);
+}
+
+template <typename V, uint32_t T>
+__global__ void __launch_bounds__(kThreads) expectation_kernel(ExpArgs a) {
+    __shared__ double red[2][kThreads];
+    const V* s = static_cast<const V*>(a.state);
+    // A fixed grid-stride assignment (coalesced; the same for every run).
+    const uint64_t total_threads = uint64_t(a.blocks) * kThreads;
+    const uint64_t tid = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+    double accr[T], acci[T];
+#pragma unroll
+    for (uint32_t t = 0; t < T; ++t) accr[t] = acci[t] = 0.0;
+
+    if (a.x == 0) {
+        for (uint64_t m = tid; m < a.pairs; m += total_threads) {
+            double ar, ai;
+            load_amp(s, m, ar, ai);
+            // conj(a) * a: real ar*ar + ai*ai, imaginary ar*ai - ai*ar (exactly 0).
+            const double pr = ar * ar + ai * ai;
+            const double pi = ar * ai - ai * ar;
+#pragma unroll
+            for (uint32_t t = 0; t < T; ++t) {
+                if (t >= a.terms) break;
+                const bool neg = (__popcll(a.z[t] & m) & 1) != 0;
+                accr[t] = neg ? accr[t] - pr : accr[t] + pr;
+                acci[t] = neg ? acci[t] - pi : acci[t] + pi;
+            }
+        }
+    } else {
+        const uint64_t lowm = (1ull << a.h) - 1ull;
+        for (uint64_t r = tid; r < a.pairs; r += total_threads) {
+            const uint64_t m = ((r & ~lowm) << 1) | (r & lowm);
+            const uint64_t mp = m ^ a.x;
+            double ar, ai, br, bi;
+            load_amp(s, m, ar, ai);
+            load_amp(s, mp, br, bi);
+            // p = conj(a) * b
+            const double pr = ar * br + ai * bi;
+            const double pi = ar * bi - ai * br;
+#pragma unroll
+            for (uint32_t t = 0; t < T; ++t) {
+                if (t >= a.terms) break;
+                const bool neg_mp = (__popcll(a.z[t] & mp) & 1) != 0;  // s(m') for p
+                const bool neg_m = (__popcll(a.z[t] & m) & 1) != 0;    // s(m) for conj(p)
+                accr[t] = neg_mp ? accr[t] - pr : accr[t] + pr;
+                acci[t] = neg_mp ? acci[t] - pi : acci[t] + pi;
+                accr[t] = neg_m ? accr[t] - pr : accr[t] + pr;
+                acci[t] = neg_m ? acci[t] + pi : acci[t] - pi;
+            }
+        }
+    }
+    // Fixed-order tree over the workgroup, one term at a time.
+    for (uint32_t t = 0; t < a.terms && t < T; ++t) {
+        double vr = 0.0, vi = 0.0;
+#pragma unroll
+        for (uint32_t u = 0; u < T; ++u)
+            if (u == t) { vr = accr[u]; vi = acci[u]; }
+        red[0][threadIdx.x] = vr;
+        red[1][threadIdx.x] = vi;
+        __syncthreads();
+        for (int stride = kThreads / 2; stride > 0; stride >>= 1) {
+            if (threadIdx.x < stride) {
+                red[0][threadIdx.x] = red[0][threadIdx.x] + red[0][threadIdx.x + stride];
+                red[1][threadIdx.x] = red[1][threadIdx.x] + red[1][threadIdx.x + stride];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            a.partials[(uint64_t(t) * a.blocks + blockIdx.x) * 2] = red[0][0];
+            a.partials[(uint64_t(t) * a.blocks + blockIdx.x) * 2 + 1] = red[1][0];
+        }
+        __syncthreads();
+    }
+}
+
+// One workgroup per term: the block partials in a fixed tree order, then the
+// phase i^popcount(x & z) as a complex product (reference dense_kernels.cpp:568).
+__global__ void __launch_bounds__(kThreads) expectation_finish_kernel(const double* partials, uint32_t blocks,
+                                                                      const int32_t* phase_k, double* out) {
+    __shared__ double red[2][kThreads];
+    const uint32_t t = blockIdx.x;
+    double vr = 0.0, vi = 0.0;
+    for (uint32_t b = threadIdx.x; b < blocks; b += kThreads) {
+        vr = vr + partials[(uint64_t(t) * blocks + b) * 2];
+        vi = vi + partials[(uint64_t(t) * blocks + b) * 2 + 1];
+    }
+    red[0][threadIdx.x] = vr;
+    red[1][threadIdx.x] = vi;
+    __syncthreads();
+    for (int stride = kThreads / 2; stride > 0; stride >>= 1) {
+        if (threadIdx.x < stride) {
+            red[0][threadIdx.x] = red[0][threadIdx.x] + red[0][threadIdx.x + stride];
+            red[1][threadIdx.x] = red[1][threadIdx.x] + red[1][threadIdx.x + stride];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int k = phase_k[t] & 3;
+        const double fr = k == 0 ? 1.0 : (k == 2 ? -1.0 : 0.0);
+        const double fi = k == 1 ? 1.0 : (k == 3 ? -1.0 : 0.0);
+        const double sr = red[0][0], si = red[1][0];
+        out[2 * t] = fr * sr - fi * si;
+        out[2 * t + 1] = fr * si + fi * sr;
+    }
+}
+
+template <typename V>
+void run_group(const void* state, uint32_t n, uint64_t x, const std::vector<uint32_t>& idx, const uint64_t* z,
+               double* out_all) {
+    Context& ctx = context();
+    const uint32_t terms = static_cast<uint32_t>(idx.size());
+    ExpArgs a{};
+    a.state = state;
+    a.x = x;
+    a.h = x ? 63u - static_cast<uint32_t>(__builtin_clzll(x)) : 0u;
+    a.pairs = x ? (uint64_t{1} << (n - 1)) : (uint64_t{1} << n);
+    a.terms = terms;
+    a.blocks = static_cast<uint32_t>(std::min<uint64_t>(kMaxBlocks, std::max<uint64_t>(1, a.pairs / (kThreads * 16))));
+    for (uint32_t t = 0; t < terms; ++t) a.z[t] = z[idx[t]];
+    DeviceBuffer partials(uint64_t(a.blocks) * terms * 2 * sizeof(double));
+    DeviceBuffer res(uint64_t(terms) * 2 * sizeof(double) + terms * sizeof(int32_t));
+    std::vector<int32_t> phase(terms);
+    for (uint32_t t = 0; t < terms; ++t) phase[t] = __builtin_popcountll(x & a.z[t]);
+    int32_t* dphase = reinterpret_cast<int32_t*>(res.as<double>() + 2 * terms);
+    upload(dphase, phase.data(), terms * sizeof(int32_t));
+    a.partials = partials.as<double>();
+    if (terms <= 1)
+        hipLaunchKernelGGL((expectation_kernel<V, 1>), dim3(a.blocks), dim3(kThreads), 0, ctx.stream, a);
+    else if (terms <= 4)
+        hipLaunchKernelGGL((expectation_kernel<V, 4>), dim3(a.blocks), dim3(kThreads), 0, ctx.stream, a);
+    else
+        hipLaunchKernelGGL((expectation_kernel<V, kTermsPerLaunch>), dim3(a.blocks), dim3(kThreads), 0, ctx.stream, a);
+    MH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(expectation_finish_kernel, dim3(terms), dim3(kThreads), 0, ctx.stream,
+                       static_cast<const double*>(partials.get()), a.blocks, static_cast<const int32_t*>(dphase),
+                       res.as<double>());
+    MH_HIP(hipGetLastError());
+    std::vector<double> host(2 * terms);
+    download(host.data(), res.get(), host.size() * sizeof(double));
+    for (uint32_t t = 0; t < terms; ++t) {
+        out_all[2 * idx[t]] = host[2 * t];
+        out_all[2 * idx[t] + 1] = host[2 * t + 1];
+    }
+}
+
+}  // namespace
+
+void expectation_terms(const void* state, uint32_t n, int32_t precision, const uint64_t* x, const uint64_t* z,
+                       uint32_t terms, cplx* out) {
+    amp_bytes(precision);
+    if (n == 0 || n > 40) fail(MH_ERR_INVALID, "expectation: num_qubits must be 1..40");
+    const uint64_t limit = n == 64 ? ~0ull : ((uint64_t{1} << n) - 1);
+    std::map<uint64_t, std::vector<uint32_t>> groups;
+    for (uint32_t t = 0; t < terms; ++t) {
+        if ((x[t] & ~limit) || (z[t] & ~limit)) fail(MH_ERR_INVALID, "a Pauli mask names a qubit beyond the state");
+        groups[x[t]].push_back(t);
+    }
+    double* o = reinterpret_cast<double*>(out);
+    for (auto& [xm, members] : groups) {
+        for (std::size_t at = 0; at < members.size(); at += kTermsPerLaunch) {
+            std::vector<uint32_t> part(members.begin() + at,
+                                       members.begin() + std::min(members.size(), at + kTermsPerLaunch));
+            if (precision == MH_C128)
+                run_group<double2>(state, n, xm, part, z, o);
+            else
+                run_group<float2>(state, n, xm, part, z, o);
+        }
+    }
+}
+
+}  // namespace mh
+
+using namespace mh;
+
+extern "C" int mh_expectation(const void* state, uint32_t num_qubits, int32_t precision, const uint64_t* x_masks,
+                              const uint64_t* z_masks, uint32_t num_terms, double* out) {
+    return guarded([&] {
+        expectation_terms(state, num_qubits, precision, x_masks, z_masks, num_terms, reinterpret_cast<cplx*>(out));
+    });
+}

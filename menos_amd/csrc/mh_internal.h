@@ -1,0 +1,161 @@
+// Internal declarations shared by the engine's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "menos_hip.h"
+
+namespace mh {
+
+using cplx = std::complex<double>;
+
+// Errors raised inside the library and turned into statuses at the C boundary.
+struct Error : std::runtime_error {
+    mh_status status;
+    Error(mh_status s, const std::string& what) : std::runtime_error(what), status(s) {}
+};
+
+[[noreturn]] void fail(mh_status status, const std::string& what);
+void check_hip(hipError_t e, const char* what);
+#define MH_HIP(call) ::mh::check_hip((call), #call)
+
+void set_last_error(const std::string& message);
+
+// Runs f, mapping exceptions to statuses (the C boundary of every entry point).
+template <typename F>
+int guarded(F&& f) noexcept {
+    try {
+        f();
+        return MH_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return MH_ERR_RESOURCE;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return MH_ERR_INTERNAL;
+    }
+}
+
+// The library's device context: one stream of default priority on device 0,
+// created on first use.
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t props{};
+};
+Context& context();
+void synchronize();
+
+std::size_t amp_bytes(int32_t precision);
+
+// ---- memory pool (device.hip)
+void* pool_alloc(uint64_t bytes);
+void pool_free(void* p) noexcept;
+uint64_t pool_reusable_bytes();
+void pool_trim();
+uint64_t pool_live_bytes();
+
+// A device buffer from the pool, released on destruction.
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(uint64_t bytes) : ptr_(bytes ? pool_alloc(bytes) : nullptr), bytes_(bytes) {}
+    ~DeviceBuffer() { reset(); }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_) { o.ptr_ = nullptr; o.bytes_ = 0; }
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
+        if (this != &o) { reset(); ptr_ = o.ptr_; bytes_ = o.bytes_; o.ptr_ = nullptr; o.bytes_ = 0; }
+        return *this;
+    }
+    void reset() noexcept { if (ptr_) pool_free(ptr_); ptr_ = nullptr; bytes_ = 0; }
+    void* get() const { return ptr_; }
+    uint64_t bytes() const { return bytes_; }
+    template <typename T> T* as() const { return static_cast<T*>(ptr_); }
+private:
+    void* ptr_ = nullptr;
+    uint64_t bytes_ = 0;
+};
+
+// Pinned host staging for small uploads (matrices, pass programs).
+void upload(void* device_dst, const void* host_src, std::size_t bytes);
+void download(void* host_dst, const void* device_src, std::size_t bytes);
+
+// ---- kernels (gates.hip)
+void launch_init(void* state, uint32_t n, int32_t precision, double re, double im);
+// One gate, dispatched (diagonal / k=1 / k=2 / generic) or generic only.
+void apply_gate(void* state, uint32_t n, int32_t precision, const cplx* matrix, const uint32_t* operands,
+                uint32_t k, bool generic_only);
+
+// A block ready to run: matrix (row-major, 4^k entries) and operands in the
+// matrix's operand order.
+struct Block {
+    std::vector<uint32_t> operands;
+    std::vector<cplx> matrix;
+    bool diagonal = false;
+};
+
+struct PassStats {
+    uint32_t passes = 0;
+    std::vector<float> pass_ms;  // filled when timing is requested
+};
+
+// Applies the blocks in order, grouped into window passes.
+void apply_blocks(void* state, uint32_t n, int32_t precision, const std::vector<Block>& blocks,
+                  uint32_t window_qubits, bool time_passes, PassStats* stats);
+
+// The default window: the widest that fits the device's LDS (12 qubits at
+// complex128, 13 at complex64 on gfx950).
+uint32_t window_default(int32_t precision) noexcept;
+
+// Number of HBM passes the schedule makes for these blocks.
+uint32_t count_passes(uint32_t n, const std::vector<Block>& blocks, uint32_t window_qubits);
+
+bool matrix_is_diagonal(const cplx* m, std::size_t dim);
+
+// ---- expectation (expectation.hip)
+void expectation_terms(const void* state, uint32_t n, int32_t precision, const uint64_t* x, const uint64_t* z,
+                       uint32_t terms, cplx* out);
+
+// ---- sampler (sampler.hip)
+void sample_sorted(const void* state, uint32_t n, int32_t precision, const double* sorted_u, uint64_t shots,
+                   uint64_t* basis_out);
+
+// ---- host IR (host_ir.cpp)
+struct CircuitView {
+    const mh_circuit* c;
+    uint32_t num_ops() const { return c->num_ops; }
+    int32_t opcode(uint32_t i) const { return c->opcodes[i]; }
+    const uint32_t* qubits(uint32_t i) const { return c->qubits + c->qubit_offsets[i]; }
+    uint32_t num_qubits_of(uint32_t i) const { return c->qubit_offsets[i + 1] - c->qubit_offsets[i]; }
+    const double* params(uint32_t i) const { return c->params + c->param_offsets[i]; }
+    uint32_t num_params(uint32_t i) const { return c->param_offsets[i + 1] - c->param_offsets[i]; }
+};
+
+constexpr int32_t kOpUnitary = 21;
+constexpr int32_t kOpMeasure = 22;
+
+std::vector<cplx> gate_matrix(int32_t opcode, const double* params, uint32_t num_params);
+std::vector<cplx> op_matrix(const CircuitView& cv, uint32_t i);
+bool op_is_diagonal_for_fusion(const CircuitView& cv, uint32_t i);
+void validate_circuit(const mh_circuit* c);
+
+struct FusionPlan {
+    std::vector<uint32_t> order, block_begin, block_qubits, qubits_begin;
+    uint64_t fused_blocks = 0, gates_absorbed = 0;
+    uint32_t block_count() const { return block_begin.empty() ? 0 : static_cast<uint32_t>(block_begin.size() - 1); }
+};
+
+FusionPlan plan_fusion(const CircuitView& cv, int32_t fusion, uint32_t width);
+std::vector<Block> build_blocks(const CircuitView& cv, const FusionPlan& plan);
+
+}  // namespace mh

@@ -1,0 +1,322 @@
+"""ctypes bindings of libmenos_hip.so (the C ABI in include/menos_hip.h).
+
+The library is built in-tree by ``menos_amd.build`` and loaded from this
+package's directory; loading fails loudly if it is missing (there is no other
+implementation to fall back to).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import workloads as W
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmenos_hip.so")
+
+C128, C64 = 0, 1
+FUSE_NONE, FUSE_CONTAINMENT, FUSE_WIDTH = 0, 1, 2
+MH_OK, MH_ERR_INVALID, MH_ERR_RESOURCE, MH_ERR_INTERNAL, MH_ERR_DEVICE = range(5)
+
+# Every entry point include/menos_hip.h declares.
+EXPORTS = (
+    "mh_last_error", "mh_device_query", "mh_device_available_bytes", "mh_pool_alloc", "mh_pool_free",
+    "mh_pool_reusable_bytes", "mh_pool_trim", "mh_state_init", "mh_apply_matrix", "mh_apply_matrix_generic",
+    "mh_apply_blocks", "mh_expectation", "mh_sample", "mh_sample_unsorted", "mh_copy_to_host",
+    "mh_copy_from_host", "mh_state_digest", "mh_engine_estimate", "mh_engine_execute", "mh_plan_fusion",
+    "mh_gate_matrix", "mh_calibrate",
+)
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_f32p = ctypes.POINTER(ctypes.c_float)
+
+
+class HipError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[status {status}] {message}")
+        self.status = status
+
+
+class ResourceExceeded(HipError):
+    """Over the device budget or out of device memory (reference ResourceExceeded)."""
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 128), ("arch", ctypes.c_char * 64), ("compute_units", ctypes.c_int32),
+                ("warp_size", ctypes.c_int32), ("max_threads_per_block", ctypes.c_int32),
+                ("lds_bytes_per_block", ctypes.c_int32), ("clock_khz", ctypes.c_int32),
+                ("memory_clock_khz", ctypes.c_int32), ("memory_bus_width", ctypes.c_int32),
+                ("total_bytes", ctypes.c_uint64), ("free_bytes", ctypes.c_uint64)]
+
+
+class CCircuit(ctypes.Structure):
+    _fields_ = [("num_qubits", ctypes.c_uint32), ("num_ops", ctypes.c_uint32), ("opcodes", _i32p),
+                ("qubit_offsets", _u32p), ("qubits", _u32p), ("param_offsets", _u32p), ("params", _f64p)]
+
+
+class CRequest(ctypes.Structure):
+    _fields_ = [("circuit", ctypes.POINTER(CCircuit)), ("precision", ctypes.c_int32), ("fusion", ctypes.c_int32),
+                ("fusion_width", ctypes.c_uint32), ("window_qubits", ctypes.c_uint32),
+                ("global_phase", ctypes.c_double), ("shots", ctypes.c_uint64), ("seed", ctypes.c_uint64),
+                ("target_qubits", _u32p), ("target_clbits", _u32p), ("num_targets", ctypes.c_uint32),
+                ("bitstring_width", ctypes.c_uint32), ("num_terms", ctypes.c_uint32), ("x_masks", _u64p),
+                ("z_masks", _u64p), ("budget_bytes", ctypes.c_uint64), ("want_statevector", ctypes.c_int32),
+                ("record_pass_times", ctypes.c_int32)]
+
+
+class CResult(ctypes.Structure):
+    _fields_ = [("statevector", ctypes.c_void_p), ("outcomes", _u64p), ("counts", _u64p),
+                ("num_outcomes", ctypes.c_uint64), ("expectation_terms", _f64p), ("pass_ms", _f32p),
+                ("pass_ms_capacity", ctypes.c_uint32), ("num_blocks", ctypes.c_uint32),
+                ("num_passes", ctypes.c_uint32), ("fused_blocks", ctypes.c_uint64),
+                ("gates_absorbed", ctypes.c_uint64), ("actual_bytes", ctypes.c_uint64),
+                ("peak_device_bytes", ctypes.c_uint64), ("execute_seconds", ctypes.c_double),
+                ("gate_seconds", ctypes.c_double), ("sample_seconds", ctypes.c_double),
+                ("expectation_seconds", ctypes.c_double)]
+
+
+class CCost(ctypes.Structure):
+    _fields_ = [("feasible", ctypes.c_int32), ("infeasible_reason", ctypes.c_char * 256),
+                ("device_bytes", ctypes.c_uint64), ("host_bytes", ctypes.c_uint64), ("work_units", ctypes.c_uint64),
+                ("blocks", ctypes.c_uint32), ("passes", ctypes.c_uint32), ("launches", ctypes.c_uint64),
+                ("h2d_bytes", ctypes.c_uint64), ("d2h_bytes", ctypes.c_uint64), ("seconds", ctypes.c_double)]
+
+
+class Calibration(ctypes.Structure):
+    _fields_ = [("hbm_bytes_per_second", ctypes.c_double), ("pass_bytes_per_second", ctypes.c_double),
+                ("launch_seconds", ctypes.c_double), ("h2d_bytes_per_second", ctypes.c_double),
+                ("d2h_bytes_per_second", ctypes.c_double), ("fp64_ops_per_second", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is not built: run `python -m menos_amd.build`")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.mh_last_error.restype = ctypes.c_char_p
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != MH_OK:
+        msg = lib().mh_last_error().decode(errors="replace")
+        raise (ResourceExceeded if status == MH_ERR_RESOURCE else HipError)(status, msg)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None and a.size else ctypes.cast(None, t)
+
+
+def device_info() -> DeviceInfo:
+    info = DeviceInfo()
+    check(lib().mh_device_query(ctypes.byref(info)))
+    return info
+
+
+def amp_dtype(precision: int):
+    return np.complex128 if precision == C128 else np.complex64
+
+
+class DeviceState:
+    """A statevector in device memory, allocated from the engine's pool."""
+
+    def __init__(self, num_qubits: int, precision: int = C128):
+        self.n, self.precision = num_qubits, precision
+        self.nbytes = (1 << num_qubits) * (16 if precision == C128 else 8)
+        p = ctypes.c_void_p()
+        check(lib().mh_pool_alloc(ctypes.c_uint64(self.nbytes), ctypes.byref(p)))
+        self.ptr = p
+
+    def free(self):
+        if self.ptr:
+            lib().mh_pool_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def init(self, phase: complex = 1.0):
+        check(lib().mh_state_init(self.ptr, self.n, self.precision, ctypes.c_double(phase.real),
+                                  ctypes.c_double(complex(phase).imag)))
+
+    def upload(self, host: np.ndarray):
+        h = np.ascontiguousarray(host, amp_dtype(self.precision))
+        check(lib().mh_copy_from_host(self.ptr, self.n, self.precision, h.ctypes.data_as(ctypes.c_void_p)))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(1 << self.n, amp_dtype(self.precision))
+        check(lib().mh_copy_to_host(self.ptr, self.n, self.precision, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def digest(self) -> int:
+        h = ctypes.c_uint64()
+        check(lib().mh_state_digest(self.ptr, self.n, self.precision, ctypes.byref(h)))
+        return h.value
+
+    def apply_matrix(self, matrix, operands, generic=False):
+        m = np.ascontiguousarray(matrix, np.complex128)
+        ops = np.asarray(operands, np.uint32)
+        fn = lib().mh_apply_matrix_generic if generic else lib().mh_apply_matrix
+        check(fn(self.ptr, self.n, self.precision, m.ctypes.data_as(_f64p), _ptr(ops, _u32p), len(ops)))
+
+    def apply_blocks(self, blocks, window_qubits=0) -> int:
+        """blocks: [(matrix, operands)]. Returns the HBM pass count."""
+        offs, ops, moffs, mats = [0], [], [], []
+        at = 0
+        for m, q in blocks:
+            ops.extend(q)
+            offs.append(len(ops))
+            moffs.append(at)
+            flat = np.ascontiguousarray(m, np.complex128).reshape(-1)
+            mats.append(flat)
+            at += flat.size
+        offs = np.array(offs, np.uint32)
+        ops = np.array(ops, np.uint32)
+        moffs = np.array(moffs, np.uint64)
+        mats = np.concatenate(mats) if mats else np.zeros(1, np.complex128)
+        passes = ctypes.c_uint32()
+        check(lib().mh_apply_blocks(self.ptr, self.n, self.precision, len(blocks), _ptr(offs, _u32p),
+                                    _ptr(ops, _u32p), _ptr(moffs, _u64p), mats.ctypes.data_as(_f64p),
+                                    window_qubits, ctypes.byref(passes)))
+        return passes.value
+
+    def expectation(self, terms) -> np.ndarray:
+        xs = np.array([t[0] for t in terms], np.uint64)
+        zs = np.array([t[1] for t in terms], np.uint64)
+        out = np.zeros(2 * max(len(terms), 1), np.float64)
+        check(lib().mh_expectation(self.ptr, self.n, self.precision, _ptr(xs, _u64p), _ptr(zs, _u64p), len(terms),
+                                   out.ctypes.data_as(_f64p)))
+        return (out[0::2] + 1j * out[1::2])[:len(terms)]
+
+    def sample(self, sorted_uniforms) -> np.ndarray:
+        u = np.ascontiguousarray(sorted_uniforms, np.float64)
+        out = np.zeros(max(u.size, 1), np.uint64)
+        check(lib().mh_sample(self.ptr, self.n, self.precision, _ptr(u, _f64p), ctypes.c_uint64(u.size),
+                              out.ctypes.data_as(_u64p)))
+        return out[:u.size]
+
+    def sample_unsorted(self, uniforms) -> np.ndarray:
+        u = np.ascontiguousarray(uniforms, np.float64)
+        out = np.zeros(max(u.size, 1), np.uint64)
+        check(lib().mh_sample_unsorted(self.ptr, self.n, self.precision, _ptr(u, _f64p), ctypes.c_uint64(u.size),
+                                       out.ctypes.data_as(_u64p)))
+        return out[:u.size]
+
+
+def gate_matrix(opcode: int, params=()) -> np.ndarray:
+    p = np.asarray(params, np.float64)
+    out = np.zeros(64, np.complex128)
+    k = ctypes.c_uint32()
+    check(lib().mh_gate_matrix(ctypes.c_int32(opcode), _ptr(p, _f64p), out.ctypes.data_as(_f64p), ctypes.byref(k)))
+    d = 1 << k.value
+    return out[:d * d].reshape(d, d)
+
+
+class _CircuitKeep:
+    def __init__(self, circuit: W.Circuit):
+        self.arrays = circuit.flat()
+        op, qo, qs, po, ps = self.arrays
+        self.c = CCircuit(circuit.num_qubits, len(op), _ptr(op, _i32p), _ptr(qo, _u32p), _ptr(qs, _u32p),
+                          _ptr(po, _u32p), _ptr(ps, _f64p))
+
+
+def plan_fusion(circuit: W.Circuit, fusion=FUSE_CONTAINMENT, width=0):
+    """(order, block_begin, block_qubits, qubits_begin, fused_blocks, gates_absorbed)."""
+    keep = _CircuitKeep(circuit)
+    n_ops = len(circuit.ops)
+    nq = sum(len(o[1]) for o in circuit.ops) + 1
+    order = np.zeros(max(n_ops, 1), np.uint32)
+    bb = np.zeros(n_ops + 2, np.uint32)
+    bq = np.zeros(nq, np.uint32)
+    qb = np.zeros(n_ops + 2, np.uint32)
+    nb = ctypes.c_uint32()
+    fb, ga = ctypes.c_uint64(), ctypes.c_uint64()
+    check(lib().mh_plan_fusion(ctypes.byref(keep.c), fusion, width, ctypes.byref(nb), _ptr(order, _u32p),
+                               _ptr(bb, _u32p), _ptr(bq, _u32p), _ptr(qb, _u32p), ctypes.byref(fb),
+                               ctypes.byref(ga)))
+    b = nb.value
+    return order[:n_ops], bb[:b + 1], bq[:qb[b]], qb[:b + 1], fb.value, ga.value
+
+
+class Result:
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+class HipDenseEngine:
+    """The {Dense, Hip, InMemory} engine (SimulationEngine::estimate / execute)."""
+
+    name = "dense_hip"
+
+    @staticmethod
+    def _request(circuit, keep, *, precision, fusion, fusion_width, window_qubits, shots, seed, targets, width,
+                 terms, budget_bytes, statevector, record_pass_times):
+        targets = circuit.measure_targets() if targets is None else targets
+        width = circuit.num_clbits if width is None else width
+        tq = np.array([t[0] for t in targets], np.uint32)
+        tc = np.array([t[1] for t in targets], np.uint32)
+        xs = np.array([t[0] for t in terms], np.uint64)
+        zs = np.array([t[1] for t in terms], np.uint64)
+        req = CRequest(ctypes.pointer(keep.c), precision, fusion, fusion_width, window_qubits, circuit.global_phase,
+                       shots, seed, _ptr(tq, _u32p), _ptr(tc, _u32p), len(targets), width, len(terms),
+                       _ptr(xs, _u64p), _ptr(zs, _u64p), budget_bytes, int(bool(statevector)),
+                       int(bool(record_pass_times)))
+        return req, (tq, tc, xs, zs)
+
+    def estimate(self, circuit: W.Circuit, *, precision=C128, fusion=FUSE_CONTAINMENT, fusion_width=0,
+                 window_qubits=0, shots=0, seed=0, targets=None, width=None, terms=(), budget_bytes=0,
+                 statevector=False):
+        keep = _CircuitKeep(circuit)
+        req, hold = self._request(circuit, keep, precision=precision, fusion=fusion, fusion_width=fusion_width,
+                                  window_qubits=window_qubits, shots=shots, seed=seed, targets=targets, width=width,
+                                  terms=terms, budget_bytes=budget_bytes, statevector=statevector,
+                                  record_pass_times=False)
+        cost = CCost()
+        check(lib().mh_engine_estimate(ctypes.byref(req), ctypes.byref(cost)))
+        return cost
+
+    def execute(self, circuit: W.Circuit, *, precision=C128, fusion=FUSE_CONTAINMENT, fusion_width=0,
+                window_qubits=0, shots=0, seed=0, targets=None, width=None, terms=(), budget_bytes=0,
+                statevector=True, record_pass_times=False, max_passes=4096) -> Result:
+        keep = _CircuitKeep(circuit)
+        req, hold = self._request(circuit, keep, precision=precision, fusion=fusion, fusion_width=fusion_width,
+                                  window_qubits=window_qubits, shots=shots, seed=seed, targets=targets, width=width,
+                                  terms=terms, budget_bytes=budget_bytes, statevector=statevector,
+                                  record_pass_times=record_pass_times)
+        sv = np.zeros(1 << circuit.num_qubits, amp_dtype(precision)) if statevector else None
+        outc = np.zeros(max(shots, 1), np.uint64)
+        cnt = np.zeros(max(shots, 1), np.uint64)
+        ex = np.zeros(2 * max(len(terms), 1), np.float64)
+        pms = np.zeros(max_passes, np.float32)
+        res = CResult(sv.ctypes.data_as(ctypes.c_void_p) if sv is not None else None,
+                      outc.ctypes.data_as(_u64p), cnt.ctypes.data_as(_u64p), 0, ex.ctypes.data_as(_f64p),
+                      pms.ctypes.data_as(_f32p), max_passes)
+        check(lib().mh_engine_execute(ctypes.byref(req), ctypes.byref(res)))
+        counts = {int(outc[i]): int(cnt[i]) for i in range(res.num_outcomes)} if shots else None
+        return Result(statevector=sv, counts=counts,
+                      expectation=(ex[0::2] + 1j * ex[1::2])[:len(terms)] if terms else None,
+                      pass_ms=pms[:min(res.num_passes, max_passes)] if record_pass_times else None,
+                      num_blocks=res.num_blocks, num_passes=res.num_passes, fused_blocks=res.fused_blocks,
+                      gates_absorbed=res.gates_absorbed, actual_bytes=res.actual_bytes,
+                      peak_device_bytes=res.peak_device_bytes, execute_seconds=res.execute_seconds,
+                      gate_seconds=res.gate_seconds, sample_seconds=res.sample_seconds,
+                      expectation_seconds=res.expectation_seconds)
+
+
+def calibrate() -> Calibration:
+    c = Calibration()
+    check(lib().mh_calibrate(ctypes.byref(c)))
+    return c
