@@ -1,0 +1,153 @@
+"""Benchmark: the dense random circuit at 33 qubits, complex128, on one MI355X.
+
+A step is one execution of the whole circuit (16 layers; each layer U3 on every
+qubit then CZ on the brick pattern) on the 128 GiB state already resident in
+HBM: initialisation plus every gate pass. `value` is the achieved HBM bandwidth
+of the gate passes: the algorithmic bytes they move (one read and one write of
+the state per pass, 2 * 16 * 2^33 B) divided by their device time, measured
+with HIP events on the engine's stream around every pass.
+
+Prints one JSON line. Progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0  # MI355X datasheet
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(qubits: int, layers: int, threads: int) -> dict:
+    """The reference CPU engine (oracle/_ref, DenseEngine::execute) on a bounded
+    sample of the same workload, as GB/s of its own gate passes (one read and
+    one write of the state per plan block)."""
+    import oracle
+    from menos_amd import workloads as W
+    c = W.random_circuit(qubits, layers, 2025)
+    plan = oracle.Oracle().plan_fusion(c)
+    blocks = plan.num_blocks
+    if oracle.Reference.available():
+        _, _, _, secs = oracle.Reference().execute(c, fuse_gates=True, statevector=False, threads=threads)
+        kind = "reference"
+    else:
+        t0 = time.perf_counter()
+        oracle.Oracle().execute(c, fuse_gates=True, statevector=False, threads=threads)
+        secs = time.perf_counter() - t0
+        kind = "port"
+    algo = blocks * 2 * 16 * (1 << qubits)
+    return {"value": round(algo / secs / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": kind,
+            "sample": f"random circuit {qubits} qubits complex128, {layers} layers ({blocks} gate passes), "
+                      f"{secs:.2f} s on {threads} threads"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=33)
+    ap.add_argument("--layers", type=int, default=16)
+    ap.add_argument("--precision", choices=["c128", "c64"], default="c128")
+    ap.add_argument("--fusion", choices=["none", "containment", "width"], default="containment")
+    ap.add_argument("--fusion-width", type=int, default=0)
+    ap.add_argument("--window", type=int, default=0)
+    ap.add_argument("--cpu-qubits", type=int, default=25)
+    ap.add_argument("--cpu-layers", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); this engine has no multi-GPU path.",
+              file=sys.stderr)
+        return 2
+
+    from menos_amd import hip as H
+    from menos_amd import workloads as W
+
+    info = H.device_info()
+    log(f"device {info.name.decode()} {info.arch.decode()} CUs={info.compute_units} "
+        f"HBM={info.total_bytes / 2**30:.0f} GiB free={info.free_bytes / 2**30:.0f} GiB")
+    precision = H.C128 if args.precision == "c128" else H.C64
+    amp = 16 if precision == H.C128 else 8
+    fusion = {"none": H.FUSE_NONE, "containment": H.FUSE_CONTAINMENT, "width": H.FUSE_WIDTH}[args.fusion]
+    n = args.qubits
+    circuit = W.random_circuit(n, args.layers, 2025)
+    eng = H.HipDenseEngine()
+    est = eng.estimate(circuit, precision=precision, fusion=fusion, fusion_width=args.fusion_width,
+                       window_qubits=args.window)
+    if not est.feasible:
+        log(f"infeasible: {est.infeasible_reason.decode()}")
+        return 1
+    log(f"{n} qubits {args.precision}, {len(circuit.ops)} gates -> {est.blocks} blocks, {est.passes} passes, "
+        f"state {est.device_bytes / 2**30:.1f} GiB")
+
+    def step():
+        return eng.execute(circuit, precision=precision, fusion=fusion, fusion_width=args.fusion_width,
+                           window_qubits=args.window, statevector=False, record_pass_times=True)
+
+    for i in range(args.warmup):
+        r = step()
+        log(f"warmup {i}: execute {r.execute_seconds:.3f} s, gate passes {r.gate_seconds:.3f} s")
+    pass_ms, gate_s, exec_s = [], 0.0, 0.0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r = step()
+        pass_ms.extend(r.pass_ms.tolist())
+        gate_s += r.gate_seconds
+        exec_s += r.execute_seconds
+        log(f"step {i}: execute {r.execute_seconds:.3f} s, {r.num_passes} passes in {r.gate_seconds:.3f} s")
+    wall = time.perf_counter() - t0
+    passes = r.num_passes
+    pass_bytes = 2 * amp * (1 << n)
+    total_pass_s = sum(pass_ms) * 1e-3
+    value = pass_bytes * len(pass_ms) / total_pass_s / 1e9
+    avg_ms = sum(pass_ms) / len(pass_ms)
+    achieved = pass_bytes / (avg_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": "HBM GB/s vs peak at 33q c128; batched circuits/sec + p99 latency at err<=1e-5",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == H.C128 else "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"dense random circuit, {n} qubits {args.precision} ({(1 << n) * amp / 2**30:.0f} GiB), "
+                        f"{args.layers} layers of U3 on every qubit + CZ brick, {args.fusion} fusion",
+            "num_qubits": n, "layers": args.layers, "precision": args.precision, "fusion": args.fusion,
+            "gates": len(circuit.ops), "blocks": r.num_blocks, "passes_per_step": passes,
+            "gate_seconds_per_step": round(gate_s / args.steps, 4),
+            "layers_per_second": round(args.layers * args.steps / gate_s, 3),
+            "window_qubits": args.window or "default",
+        },
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "kernel": "window_pass_kernel", "avg_launch_ms": round(avg_ms, 4),
+                     "algorithmic_bytes_per_launch": pass_bytes},
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: {args.cpu_qubits} qubits, {args.cpu_layers} layers, {threads} threads")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_qubits, args.cpu_layers, threads)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

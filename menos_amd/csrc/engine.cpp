@@ -9,6 +9,7 @@
 // launch-and-sync per block, and counts come from the device sampler (exact,
 // sampler.hip) instead of a copy-back and a host walk.
 #include "engine.hpp"
+#include "gate_table.h"
 
 #include <algorithm>
 #include <chrono>
@@ -73,7 +74,7 @@ Plan make_plan(const ExecutionRequest& r) {
 }
 
 uint32_t window_for(const ExecutionRequest& r) {
-    return std::min(r.window_qubits == 0 ? window_default(r.precision) : r.window_qubits, r.circuit->num_qubits);
+    return r.window_qubits == 0 ? window_default(r.precision) : r.window_qubits;
 }
 
 }  // namespace
@@ -146,7 +147,7 @@ ExecutionResult HipDenseEngine::execute(const ExecutionRequest& r) const {
     result.gates_absorbed = p.fusion.gates_absorbed;
     {
         ChargedBuffer state(sb);
-        const std::complex<double> phase = std::exp(std::complex<double>{0.0, r.global_phase});
+        const std::complex<double> phase = host_phase(r.global_phase);
         launch_init(state.get(), n, r.precision, phase.real(), phase.imag());
 
         hipEvent_t g0, g1;

@@ -79,6 +79,7 @@ struct PassArgs {
     uint32_t w;            // window qubits
     uint32_t low;          // qubits 0..low-1 are in the window
     uint32_t num_ops;
+    uint32_t mat_slots;    // LDS entries reserved for a staged k >= 3 matrix
     uint32_t high_q[kMaxHigh];  // global qubit of window bit low + j
 };
 
@@ -173,8 +174,11 @@ __global__ void __launch_bounds__(kThreads) window_pass_kernel(PassArgs args) {
     const uint32_t w = args.w;
     const uint32_t low = args.low;
     const uint32_t high = w - low;
+    // LDS: the window's amplitudes, then the staged matrix (both 16-byte
+    // aligned for ds_read/write_b128), then the offset table.
     V* lds = reinterpret_cast<V*>(lds_raw);
-    uint64_t* high_off = reinterpret_cast<uint64_t*>(lds_raw + (sizeof(V) << w));
+    V* lds_mat = lds + (1u << w);
+    uint64_t* high_off = reinterpret_cast<uint64_t*>(lds_mat + args.mat_slots);
 
     // Offsets of the high window bits: entry h is sum_j bit_j(h) << high_q[j].
     for (uint32_t h = threadIdx.x; h < (1u << high); h += kThreads) {
@@ -193,7 +197,6 @@ __global__ void __launch_bounds__(kThreads) window_pass_kernel(PassArgs args) {
     __syncthreads();
 
     const V* mats = static_cast<const V*>(args.mats);
-    V* lds_mat = reinterpret_cast<V*>(lds_raw + (sizeof(V) << w) + (sizeof(uint64_t) << high));
     for (uint32_t i = 0; i < args.num_ops; ++i) {
         const OpDesc op = args.ops[i];
         if (op.kind == 0) {
@@ -474,8 +477,8 @@ void run_passes(void* state, uint32_t n, const std::vector<Block>& blocks, uint3
         a.ops = dops.as<OpDesc>() + op_begin[p];
         a.mats = dmats.get();
         const uint32_t kmat = passes[p].kmax >= 3 ? passes[p].kmax : 0;
-        const std::size_t lds = (sizeof(V) << a.w) + (sizeof(uint64_t) << (a.w - a.low)) +
-                                (kmat ? (sizeof(V) << (2 * kmat)) : 0);
+        a.mat_slots = kmat ? (1u << (2 * kmat)) : 0u;
+        const std::size_t lds = (sizeof(V) << a.w) + sizeof(V) * a.mat_slots + (sizeof(uint64_t) << (a.w - a.low));
         const uint32_t grid = 1u << (n - a.w);
         const uint32_t kmax = passes[p].kmax;
         if (lds > ctx.props.sharedMemPerBlock)
@@ -571,7 +574,7 @@ void apply_blocks(void* state, uint32_t n, int32_t precision, const std::vector<
     const uint32_t wmax = max_window(precision);
     uint32_t w = window_qubits == 0 ? wmax : window_qubits;
     if (w > wmax) fail(MH_ERR_INVALID, "window_qubits above " + std::to_string(wmax));
-    if (w < 7) fail(MH_ERR_INVALID, "window_qubits below 7");
+    if (window_qubits != 0 && window_qubits < 7) fail(MH_ERR_INVALID, "window_qubits below 7");
     w = std::min(w, n);
     for (const Block& b : blocks) {
         if (b.operands.size() > w && !b.diagonal) fail(MH_ERR_INVALID, "a dense block wider than the window");

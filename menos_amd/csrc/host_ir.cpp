@@ -11,20 +11,17 @@
 #include <limits>
 #include <numbers>
 
+#include "gate_table.h"
 #include "mh_internal.h"
 
 namespace mh {
 
 namespace {
 
-using namespace std::complex_literals;
-
 enum : int32_t {
     kX, kY, kZ, kH, kS, kSdg, kT, kTdg, kSX, kSXdg, kRX, kRY, kRZ, kP, kU,
     kCX, kCZ, kSWAP, kCP, kRZZ, kCCX
 };
-
-constexpr double kPi = std::numbers::pi_v<double>;
 
 struct GateInfo {
     uint32_t arity;
@@ -81,77 +78,15 @@ void small_generic(cplx* state, uint32_t n, const cplx* m, const uint32_t* ops, 
 
 }  // namespace
 
-// reference src/ir/gate_table.cpp:19-206: the same std::complex expressions,
-// so the matrix bits come from the same libm calls and complex operators.
+// The named-gate matrices come from gate_table.cc, which is compiled with g++
+// (see there for why the compiler matters).
 std::vector<cplx> gate_matrix(int32_t op, const double* p, uint32_t num_params) {
     const GateInfo info = info_of(op);
     if (num_params != info.params)
         fail(MH_ERR_INVALID, "opcode " + std::to_string(op) + " takes " + std::to_string(info.params) + " parameters");
-    switch (op) {
-        case kX: return {0.0, 1.0, 1.0, 0.0};
-        case kY: return {0.0, -1.0i, 1.0i, 0.0};
-        case kZ: return {1.0, 0.0, 0.0, -1.0};
-        case kH: {
-            const double s = 1.0 / std::sqrt(2.0);
-            return {s, s, s, -s};
-        }
-        case kS: return {1.0, 0.0, 0.0, 1.0i};
-        case kSdg: return {1.0, 0.0, 0.0, -1.0i};
-        case kT: return {1.0, 0.0, 0.0, std::exp(1.0i * (kPi / 4.0))};
-        case kTdg: return {1.0, 0.0, 0.0, std::exp(-1.0i * (kPi / 4.0))};
-        case kSX: return {0.5 * (1.0 + 1.0i), 0.5 * (1.0 - 1.0i), 0.5 * (1.0 - 1.0i), 0.5 * (1.0 + 1.0i)};
-        case kSXdg: return {0.5 * (1.0 - 1.0i), 0.5 * (1.0 + 1.0i), 0.5 * (1.0 + 1.0i), 0.5 * (1.0 - 1.0i)};
-        case kRX: {
-            const double c = std::cos(p[0] / 2.0);
-            const double s = std::sin(p[0] / 2.0);
-            return {c, -1.0i * s, -1.0i * s, c};
-        }
-        case kRY: {
-            const double c = std::cos(p[0] / 2.0);
-            const double s = std::sin(p[0] / 2.0);
-            return {c, -s, s, c};
-        }
-        case kRZ: return {std::exp(-1.0i * (p[0] / 2.0)), 0.0, 0.0, std::exp(1.0i * (p[0] / 2.0))};
-        case kP: return {1.0, 0.0, 0.0, std::exp(1.0i * p[0])};
-        case kU: {
-            const double c = std::cos(p[0] / 2.0);
-            const double s = std::sin(p[0] / 2.0);
-            return {c, -std::exp(1.0i * p[2]) * s, std::exp(1.0i * p[1]) * s, std::exp(1.0i * (p[1] + p[2])) * c};
-        }
-        case kCX: {
-            std::vector<cplx> m(16, 0.0);
-            m[0] = 1.0; m[7] = 1.0; m[10] = 1.0; m[13] = 1.0;
-            return m;
-        }
-        case kCZ: {
-            std::vector<cplx> m(16, 0.0);
-            m[0] = 1.0; m[5] = 1.0; m[10] = 1.0; m[15] = -1.0;
-            return m;
-        }
-        case kSWAP: {
-            std::vector<cplx> m(16, 0.0);
-            m[0] = 1.0; m[6] = 1.0; m[9] = 1.0; m[15] = 1.0;
-            return m;
-        }
-        case kCP: {
-            std::vector<cplx> m(16, 0.0);
-            m[0] = 1.0; m[5] = 1.0; m[10] = 1.0; m[15] = std::exp(1.0i * p[0]);
-            return m;
-        }
-        case kRZZ: {
-            const cplx same = std::exp(-0.5i * p[0]);
-            const cplx diff = std::exp(0.5i * p[0]);
-            std::vector<cplx> m(16, 0.0);
-            m[0] = same; m[5] = diff; m[10] = diff; m[15] = same;
-            return m;
-        }
-        default: {  // kCCX
-            std::vector<cplx> m(64, 0.0);
-            for (int i = 0; i < 8; ++i) m[i * 8 + i] = 1.0;
-            m[3 * 8 + 3] = 0.0; m[7 * 8 + 7] = 0.0; m[3 * 8 + 7] = 1.0; m[7 * 8 + 3] = 1.0;
-            return m;
-        }
-    }
+    std::vector<cplx> m(std::size_t{1} << (2 * info.arity));
+    host_gate_matrix(op, p, m.data());
+    return m;
 }
 
 std::vector<cplx> op_matrix(const CircuitView& cv, uint32_t i) {
