@@ -61,6 +61,7 @@ def main() -> int:
     ap.add_argument("--fusion", choices=["none", "containment", "width"], default="containment")
     ap.add_argument("--fusion-width", type=int, default=0)
     ap.add_argument("--window", type=int, default=0)
+    ap.add_argument("--max-pass-cost", type=int, default=0)
     ap.add_argument("--cpu-qubits", type=int, default=25)
     ap.add_argument("--cpu-layers", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
@@ -84,7 +85,7 @@ def main() -> int:
     circuit = W.random_circuit(n, args.layers, 2025)
     eng = H.HipDenseEngine()
     est = eng.estimate(circuit, precision=precision, fusion=fusion, fusion_width=args.fusion_width,
-                       window_qubits=args.window)
+                       window_qubits=args.window, max_pass_cost=args.max_pass_cost)
     if not est.feasible:
         log(f"infeasible: {est.infeasible_reason.decode()}")
         return 1
@@ -93,7 +94,8 @@ def main() -> int:
 
     def step():
         return eng.execute(circuit, precision=precision, fusion=fusion, fusion_width=args.fusion_width,
-                           window_qubits=args.window, statevector=False, record_pass_times=True)
+                           window_qubits=args.window, statevector=False, record_pass_times=True,
+                           max_pass_cost=args.max_pass_cost)
 
     for i in range(args.warmup):
         r = step()
@@ -135,10 +137,11 @@ def main() -> int:
             "gate_seconds_per_step": round(gate_s / args.steps, 4),
             "layers_per_second": round(args.layers * args.steps / gate_s, 3),
             "window_qubits": args.window or "default",
+            "max_pass_cost": args.max_pass_cost or "default",
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-                     "kernel": "window_pass_kernel", "avg_launch_ms": round(avg_ms, 4),
+                     "kernel": "window_kernel", "avg_launch_ms": round(avg_ms, 4),
                      "algorithmic_bytes_per_launch": pass_bytes},
     }
     if not args.no_cpu:

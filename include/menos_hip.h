@@ -179,6 +179,7 @@ typedef struct mh_execution_request {
     uint64_t budget_bytes;         /* device budget; 0 = no limit */
     int32_t want_statevector;
     int32_t record_pass_times;     /* per-pass device times into the result */
+    uint32_t max_pass_cost;        /* FP operations per amplitude one HBM pass may carry; 0 = default */
 } mh_execution_request;
 
 typedef struct mh_execution_result {

@@ -104,7 +104,7 @@ extern "C" int mh_calibrate(mh_calibration* out) {
                 blocks.push_back(b);
             }
             PassStats stats;
-            const double s = best_seconds(3, [&] { apply_blocks(st.get(), n, MH_C128, blocks, 0, false, &stats); });
+            const double s = best_seconds(3, [&] { apply_blocks(st.get(), n, MH_C128, blocks, 0, false, &stats, 0); });
             c.pass_bytes_per_second = 2.0 * 16.0 * static_cast<double>(uint64_t{1} << n) * stats.passes / s;
         }
         c.launch_seconds = best_seconds(20, [&] {

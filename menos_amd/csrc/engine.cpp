@@ -88,7 +88,7 @@ CostEstimate HipDenseEngine::estimate(const ExecutionRequest& r) const noexcept 
         if (n > kMaxQubits) fail(MH_ERR_INVALID, "num_qubits above 40");
         const Plan p = make_plan(r);
         e.blocks = static_cast<uint32_t>(p.blocks.size());
-        e.passes = count_passes(n, p.blocks, window_for(r));
+        e.passes = count_passes(n, p.blocks, window_for(r), r.max_pass_cost);
         const uint64_t sb = state_bytes(n, r.precision);
         e.device_bytes = sb + scratch_bytes(n, r.shots, r.terms.size());
         e.host_bytes = r.want_statevector ? sb : 0;
@@ -155,7 +155,8 @@ ExecutionResult HipDenseEngine::execute(const ExecutionRequest& r) const {
         MH_HIP(hipEventCreate(&g1));
         MH_HIP(hipEventRecord(g0, ctx.stream));
         PassStats stats;
-        apply_blocks(state.get(), n, r.precision, p.blocks, window_for(r), r.record_pass_times, &stats);
+        apply_blocks(state.get(), n, r.precision, p.blocks, window_for(r), r.record_pass_times, &stats,
+                     r.max_pass_cost);
         MH_HIP(hipEventRecord(g1, ctx.stream));
         synchronize();
         float gate_ms = 0;
@@ -232,6 +233,7 @@ ExecutionRequest to_request(const mh_execution_request* q) {
     for (uint32_t t = 0; t < q->num_terms; ++t) r.terms.emplace_back(q->x_masks[t], q->z_masks[t]);
     r.budget_bytes = q->budget_bytes;
     r.record_pass_times = q->record_pass_times != 0;
+    r.max_pass_cost = q->max_pass_cost;
     return r;
 }
 

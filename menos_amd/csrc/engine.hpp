@@ -65,6 +65,7 @@ struct ExecutionRequest {
     std::vector<std::pair<uint64_t, uint64_t>> terms;  // (x, z)
     uint64_t budget_bytes = 0;
     bool record_pass_times = false;
+    uint32_t max_pass_cost = 0;
 };
 
 struct ExecutionResult {

@@ -1,0 +1,36 @@
+// The window-pass planner and launcher (window.hip).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "mh_internal.h"
+
+namespace mh {
+
+// Amplitudes a thread holds: 2^kWindowRegisterBits.
+constexpr uint32_t kWindowRegisterBits = 4;
+
+// Per-amplitude FP operations a pass may carry before a new one starts (ten
+// 1-qubit blocks' worth): beyond it the pass stops being bound by HBM.
+constexpr uint32_t kDefaultPassCost = 160;
+
+struct WindowPass {
+    uint64_t dense_mask = 0;      // qubits its dense blocks act on
+    uint32_t kmax = 0;            // widest dense block
+    std::vector<uint32_t> blocks; // consecutive plan blocks
+    bool generic = false;         // one block on the per-group generic kernel
+};
+
+// Groups consecutive blocks into passes over windows of w qubits.
+std::vector<WindowPass> plan_window_passes(uint32_t n, const std::vector<Block>& blocks, uint32_t w,
+                                           uint32_t max_cost);
+
+void run_window_passes(void* state, uint32_t n, int32_t precision, const std::vector<Block>& blocks, uint32_t w,
+                       uint32_t max_cost, bool time_passes, PassStats* stats);
+
+// The generic one-thread-per-group kernel for one block (gates.hip); synchronises.
+void launch_generic(void* state, uint32_t n, int32_t precision, const cplx* matrix, const uint32_t* operands,
+                    uint32_t k);
+
+}  // namespace mh

@@ -66,7 +66,7 @@ class CRequest(ctypes.Structure):
                 ("target_qubits", _u32p), ("target_clbits", _u32p), ("num_targets", ctypes.c_uint32),
                 ("bitstring_width", ctypes.c_uint32), ("num_terms", ctypes.c_uint32), ("x_masks", _u64p),
                 ("z_masks", _u64p), ("budget_bytes", ctypes.c_uint64), ("want_statevector", ctypes.c_int32),
-                ("record_pass_times", ctypes.c_int32)]
+                ("record_pass_times", ctypes.c_int32), ("max_pass_cost", ctypes.c_uint32)]
 
 
 class CResult(ctypes.Structure):
@@ -263,7 +263,7 @@ class HipDenseEngine:
 
     @staticmethod
     def _request(circuit, keep, *, precision, fusion, fusion_width, window_qubits, shots, seed, targets, width,
-                 terms, budget_bytes, statevector, record_pass_times):
+                 terms, budget_bytes, statevector, record_pass_times, max_pass_cost=0):
         targets = circuit.measure_targets() if targets is None else targets
         width = circuit.num_clbits if width is None else width
         tq = np.array([t[0] for t in targets], np.uint32)
@@ -273,29 +273,29 @@ class HipDenseEngine:
         req = CRequest(ctypes.pointer(keep.c), precision, fusion, fusion_width, window_qubits, circuit.global_phase,
                        shots, seed, _ptr(tq, _u32p), _ptr(tc, _u32p), len(targets), width, len(terms),
                        _ptr(xs, _u64p), _ptr(zs, _u64p), budget_bytes, int(bool(statevector)),
-                       int(bool(record_pass_times)))
+                       int(bool(record_pass_times)), max_pass_cost)
         return req, (tq, tc, xs, zs)
 
     def estimate(self, circuit: W.Circuit, *, precision=C128, fusion=FUSE_CONTAINMENT, fusion_width=0,
                  window_qubits=0, shots=0, seed=0, targets=None, width=None, terms=(), budget_bytes=0,
-                 statevector=False):
+                 statevector=False, max_pass_cost=0):
         keep = _CircuitKeep(circuit)
         req, hold = self._request(circuit, keep, precision=precision, fusion=fusion, fusion_width=fusion_width,
                                   window_qubits=window_qubits, shots=shots, seed=seed, targets=targets, width=width,
                                   terms=terms, budget_bytes=budget_bytes, statevector=statevector,
-                                  record_pass_times=False)
+                                  record_pass_times=False, max_pass_cost=max_pass_cost)
         cost = CCost()
         check(lib().mh_engine_estimate(ctypes.byref(req), ctypes.byref(cost)))
         return cost
 
     def execute(self, circuit: W.Circuit, *, precision=C128, fusion=FUSE_CONTAINMENT, fusion_width=0,
                 window_qubits=0, shots=0, seed=0, targets=None, width=None, terms=(), budget_bytes=0,
-                statevector=True, record_pass_times=False, max_passes=4096) -> Result:
+                statevector=True, record_pass_times=False, max_passes=4096, max_pass_cost=0) -> Result:
         keep = _CircuitKeep(circuit)
         req, hold = self._request(circuit, keep, precision=precision, fusion=fusion, fusion_width=fusion_width,
                                   window_qubits=window_qubits, shots=shots, seed=seed, targets=targets, width=width,
                                   terms=terms, budget_bytes=budget_bytes, statevector=statevector,
-                                  record_pass_times=record_pass_times)
+                                  record_pass_times=record_pass_times, max_pass_cost=max_pass_cost)
         sv = np.zeros(1 << circuit.num_qubits, amp_dtype(precision)) if statevector else None
         outc = np.zeros(max(shots, 1), np.uint64)
         cnt = np.zeros(max(shots, 1), np.uint64)
