@@ -88,7 +88,7 @@ CostEstimate HipDenseEngine::estimate(const ExecutionRequest& r) const noexcept 
         if (n > kMaxQubits) fail(MH_ERR_INVALID, "num_qubits above 40");
         const Plan p = make_plan(r);
         e.blocks = static_cast<uint32_t>(p.blocks.size());
-        e.passes = count_passes(n, p.blocks, window_for(r), r.max_pass_cost);
+        e.passes = count_passes(n, r.precision, p.blocks, window_for(r), r.max_pass_cost);
         const uint64_t sb = state_bytes(n, r.precision);
         e.device_bytes = sb + scratch_bytes(n, r.shots, r.terms.size());
         e.host_bytes = r.want_statevector ? sb : 0;

@@ -18,8 +18,6 @@ namespace mh {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr uint32_t kMaxWindowC128 = 12;  // 2^12 * 16 B = 64 KiB of LDS
-constexpr uint32_t kMaxWindowC64 = 13;   // 2^13 * 8 B = 64 KiB of LDS
 constexpr uint32_t kMaxK = 6;
 
 // ---- complex arithmetic
@@ -122,10 +120,6 @@ uint32_t grid_for(uint64_t items) {
     return static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(blocks, 1), uint64_t{1} << 20));
 }
 
-// The widest window whose amplitudes fit 64 KiB of LDS (12 qubits at
-// complex128, 13 at complex64).
-uint32_t max_window(int32_t precision) { return precision == MH_C128 ? kMaxWindowC128 : kMaxWindowC64; }
-
 template <typename R>
 void run_generic(void* state, uint32_t n, const cplx* matrix, const uint32_t* operands, uint32_t k) {
     using V = typename Vec<R>::type;
@@ -192,20 +186,21 @@ void launch_init(void* state, uint32_t n, int32_t precision, double re, double i
 
 void apply_blocks(void* state, uint32_t n, int32_t precision, const std::vector<Block>& blocks,
                   uint32_t window_qubits, bool time_passes, PassStats* stats, uint32_t max_cost) {
-    const uint32_t wmax = max_window(precision);
-    uint32_t w = window_qubits == 0 ? wmax : window_qubits;
+    const uint32_t wmax = window_width(precision);
+    const uint32_t w = window_qubits == 0 ? wmax : window_qubits;
     if (w > wmax) fail(MH_ERR_INVALID, "window_qubits above " + std::to_string(wmax));
-    if (window_qubits != 0 && window_qubits < 7) fail(MH_ERR_INVALID, "window_qubits below 7");
-    w = std::min(w, n);
+    if (w < 7) fail(MH_ERR_INVALID, "window_qubits below 7");
     for (const Block& b : blocks) check_operands(n, b.operands.data(), static_cast<uint32_t>(b.operands.size()));
     run_window_passes(state, n, precision, blocks, w, max_cost == 0 ? kDefaultPassCost : max_cost, time_passes, stats);
 }
 
-uint32_t window_default(int32_t precision) noexcept { return max_window(precision); }
+uint32_t window_default(int32_t precision) noexcept { return window_width(precision); }
 
-uint32_t count_passes(uint32_t n, const std::vector<Block>& blocks, uint32_t window_qubits, uint32_t max_cost) {
-    return static_cast<uint32_t>(
-        plan_window_passes(n, blocks, std::min(window_qubits, n), max_cost == 0 ? kDefaultPassCost : max_cost).size());
+uint32_t count_passes(uint32_t n, int32_t precision, const std::vector<Block>& blocks, uint32_t window_qubits,
+                      uint32_t max_cost) {
+    return static_cast<uint32_t>(plan_window_passes(n, precision, blocks, window_qubits == 0 ? window_width(precision) : window_qubits,
+                                                    max_cost == 0 ? kDefaultPassCost : max_cost)
+                                     .size());
 }
 
 void launch_generic(void* state, uint32_t n, int32_t precision, const cplx* matrix, const uint32_t* operands,

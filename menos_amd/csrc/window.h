@@ -8,12 +8,14 @@
 
 namespace mh {
 
-// Amplitudes a thread holds: 2^kWindowRegisterBits.
-constexpr uint32_t kWindowRegisterBits = 4;
-
-// Per-amplitude FP operations a pass may carry before a new one starts (ten
-// 1-qubit blocks' worth): beyond it the pass stops being bound by HBM.
+// Per-amplitude FP operations a pass may carry before a new one starts:
+// about the point where a pass's arithmetic takes as long as its HBM read
+// and write of the state.
 constexpr uint32_t kDefaultPassCost = 160;
+
+// Window qubits of one pass: the 6 lane bits of a wavefront plus the
+// register bits of a lane (4 at complex128, 5 at complex64).
+uint32_t window_width(int32_t precision) noexcept;
 
 struct WindowPass {
     uint64_t dense_mask = 0;      // qubits its dense blocks act on
@@ -22,9 +24,9 @@ struct WindowPass {
     bool generic = false;         // one block on the per-group generic kernel
 };
 
-// Groups consecutive blocks into passes over windows of w qubits.
-std::vector<WindowPass> plan_window_passes(uint32_t n, const std::vector<Block>& blocks, uint32_t w,
-                                           uint32_t max_cost);
+// Groups consecutive blocks into passes over windows of at most w qubits.
+std::vector<WindowPass> plan_window_passes(uint32_t n, int32_t precision, const std::vector<Block>& blocks,
+                                           uint32_t w, uint32_t max_cost);
 
 void run_window_passes(void* state, uint32_t n, int32_t precision, const std::vector<Block>& blocks, uint32_t w,
                        uint32_t max_cost, bool time_passes, PassStats* stats);

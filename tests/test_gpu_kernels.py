@@ -22,8 +22,10 @@ def _apply_both(orc, hip, state, matrix, ops, generic=False, precision=0):
 
 
 def test_every_one_qubit_operand(orc, hip):
+    """Below 10 qubits the generic kernel; from 10 on the wavefront windows
+    (every operand on each lane bit and register bit of the layout)."""
     rng = np.random.default_rng(2421)
-    for n in range(1, 11):
+    for n in range(1, 14):
         s = random_state(n, n)
         for q in range(n):
             m = random_unitary(1, rng)
@@ -33,7 +35,7 @@ def test_every_one_qubit_operand(orc, hip):
 
 def test_every_ordered_two_qubit_pair(orc, hip):
     rng = np.random.default_rng(2422)
-    for n in range(2, 9):
+    for n in (2, 3, 4, 5, 6, 7, 8, 10, 11):
         s = random_state(n, 100 + n)
         for a in range(n):
             for b in range(n):
@@ -75,9 +77,11 @@ def test_diagonal_blocks(orc, hip):
             assert got.tobytes() == want.tobytes(), (k, n, ops)
 
 
-def test_signed_zero_matrices_and_states(orc, hip):
-    """-0 entries and exact zeros in the state: outputs start at +0."""
-    n = 6
+@pytest.mark.parametrize("n", [6, 11])
+def test_signed_zero_matrices_and_states(orc, hip, n):
+    """-0 entries and exact zeros in the state: outputs start at +0 (n = 11
+    runs the window passes, which compute up to the sign of zeros and add +0
+    at the end)."""
     s = random_state(n, 3)
     s[::3] = -0.0 - 0.0j
     s[1::5] = complex(0.0, -0.0)
