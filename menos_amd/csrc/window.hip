@@ -48,15 +48,6 @@ constexpr uint32_t kNone = 255;
 constexpr uint32_t kMaxRuns = 16;
 constexpr uint32_t kMaxWindowDense = 2;  // widest dense block a window pass applies
 
-// Op codes.
-constexpr uint32_t kCodeDense1 = 0;     // + register bit J: a 1-qubit block
-constexpr uint32_t kCodeDense1Re = 8;   // + J: its entry (0, 0) has a zero imaginary part
-constexpr uint32_t kCodeDenseK = 16;    // + k: operands on register bits 0..k-1 in operand order
-constexpr uint32_t kCodeDiag = 24;
-constexpr uint32_t kCodeDiagSign = 25;  // every entry (+-1, +-0)
-constexpr uint32_t kCodeLaneSwap = 32;  // + 8 * lane bit + register bit
-constexpr uint32_t kCodeRegSwap = 96;   // + 8 * j1 + j2, j1 < j2
-
 template <typename R> struct Shape;
 template <> struct Shape<double> {
     static constexpr uint32_t kRegBits = 4, kLow = 3;
@@ -67,31 +58,61 @@ template <> struct Shape<float> {
     using V = float2;
 };
 
+// A pass is a list of bundles; a bundle is a 64-bit set of stages that run in
+// the fixed order of their bit indices (B register bits):
+//   0 sign diagonal, 1 diagonal,
+//   for each register bit J, at 2 + 8J: lane bit I <-> J exchange (I = 0..5),
+//     then a 1-qubit block on J whose entry (0, 0) is real, then any 1-qubit
+//     block on J,
+//   2 + 8B: 2-qubit block on register bits 0, 1; 3 + 8B sign diagonal,
+//   4 + 8B diagonal; 5 + 8B + p: swap of register-bit pair p.
+// A loop over bundles whose body is this fixed chain of independent stages
+// keeps every amplitude in its registers from stage to stage; a switch over
+// op kinds instead makes the compiler move the whole window between
+// registers on every op.
+struct Stages {
+    static constexpr uint32_t kSignEarly = 0, kDiagEarly = 1;
+    static constexpr uint32_t group(uint32_t j) { return 2 + 8 * j; }
+    static constexpr uint32_t lane_swap(uint32_t i, uint32_t j) { return group(j) + i; }
+    static constexpr uint32_t dense1_re(uint32_t j) { return group(j) + 6; }
+    static constexpr uint32_t dense1(uint32_t j) { return group(j) + 7; }
+    static constexpr uint32_t dense2(uint32_t rb) { return 2 + 8 * rb; }
+    static constexpr uint32_t sign_late(uint32_t rb) { return 3 + 8 * rb; }
+    static constexpr uint32_t diag_late(uint32_t rb) { return 4 + 8 * rb; }
+    static constexpr uint32_t pair_index(uint32_t j1, uint32_t j2, uint32_t rb) {
+        uint32_t p = 0;
+        for (uint32_t a = 0; a < j1; ++a) p += rb - 1 - a;
+        return p + (j2 - j1 - 1);
+    }
+    static constexpr uint32_t reg_swap(uint32_t j1, uint32_t j2, uint32_t rb) { return 5 + 8 * rb + pair_index(j1, j2, rb); }
+};
+static_assert(Stages::reg_swap(3, 4, 5) < 64, "stages of one bundle fit 64 bits");
+
+// The parameters of a stage that has them (blocks, not exchanges), in stage
+// order within a bundle.
 struct WOp {
-    uint32_t code;
     uint32_t mat;            // first entry in the matrix table
+    uint32_t num_outer;      // diagonal: operands outside the window
     uint64_t neg_mask;       // sign diagonal: bit e set when entry e is -1
     uint8_t lane_bit[8];     // diagonal: operand bit that lane bit i supplies, or kNone
     uint8_t outer_q[8];      // diagonal: global qubit of the m-th operand outside the window
     uint8_t outer_b[8];      // ... and its operand bit
     uint8_t rpart[32];       // diagonal: operand bits that register index r supplies
-    uint32_t num_outer;
-    uint32_t pad;
 };
 
 struct WPassArgs {
     void* state;
+    const uint64_t* bundles;
     const WOp* ops;
     const void* mats;
-    uint32_t num_ops;
+    uint32_t num_bundles;
     uint32_t normalize;       // add +0 to every component before the store
     uint32_t num_runs;        // runs of consecutive qubits outside the window
+    uint32_t pad;
     uint8_t run_start[kMaxRuns], run_len[kMaxRuns];
     uint8_t load_lane_q[8], load_reg_q[8];    // global qubit of each lane / register bit at the load
     uint8_t store_lane_q[8], store_reg_q[8];  // ... and at the store
 };
-
-template <uint32_t N> struct Words { uint32_t w[N]; };
 
 // The qubits outside the window, taken from the wave index (a bit deposit).
 __device__ __forceinline__ uint64_t outer_base(uint32_t wave, const WPassArgs& a) {
@@ -132,8 +153,8 @@ __device__ __forceinline__ void xchg(uint32_t& x, uint32_t& y, bool upper) {
         // same row of 16 lanes).
         constexpr int from_lower = I == 0 ? 0xB1 : I == 1 ? 0x4E : I == 2 ? 0x114 : 0x118;
         constexpr int from_upper = I == 0 ? 0xB1 : I == 1 ? 0x4E : I == 2 ? 0x104 : 0x108;
-        const uint32_t fy = __builtin_amdgcn_update_dpp(0u, y, from_lower, 0xF, 0xF, false);
-        const uint32_t fx = __builtin_amdgcn_update_dpp(0u, x, from_upper, 0xF, 0xF, false);
+        const uint32_t fy = __builtin_amdgcn_mov_dpp(y, from_lower, 0xF, 0xF, true);
+        const uint32_t fx = __builtin_amdgcn_mov_dpp(x, from_upper, 0xF, 0xF, true);
         const uint32_t nx = upper ? fy : x;
         const uint32_t ny = upper ? y : fx;
         x = nx;
@@ -141,64 +162,70 @@ __device__ __forceinline__ void xchg(uint32_t& x, uint32_t& y, bool upper) {
     }
 }
 
+template <uint32_t I>
+__device__ __forceinline__ void xchg(double& x, double& y, bool upper) {
+    uint32_t xl = static_cast<uint32_t>(__double2loint(x)), xh = static_cast<uint32_t>(__double2hiint(x));
+    uint32_t yl = static_cast<uint32_t>(__double2loint(y)), yh = static_cast<uint32_t>(__double2hiint(y));
+    xchg<I>(xl, yl, upper);
+    xchg<I>(xh, yh, upper);
+    x = __hiloint2double(static_cast<int>(xh), static_cast<int>(xl));
+    y = __hiloint2double(static_cast<int>(yh), static_cast<int>(yl));
+}
+
+template <uint32_t I>
+__device__ __forceinline__ void xchg(float& x, float& y, bool upper) {
+    uint32_t xb = __float_as_uint(x), yb = __float_as_uint(y);
+    xchg<I>(xb, yb, upper);
+    x = __uint_as_float(xb);
+    y = __uint_as_float(yb);
+}
+
 // Lane bit I <-> register bit J.
 template <typename V, uint32_t E, uint32_t I, uint32_t J>
 __device__ __forceinline__ void lane_swap(V* v, uint32_t lane) {
-    if constexpr ((1u << J) < E) {
-        constexpr uint32_t D = sizeof(V) / 4;
-        const bool upper = (lane >> I) & 1u;
+    const bool upper = (lane >> I) & 1u;
 #pragma unroll
-        for (uint32_t r = 0; r < E; ++r) {
-            if (r & (1u << J)) continue;
-            Words<D> x = __builtin_bit_cast(Words<D>, v[r]);
-            Words<D> y = __builtin_bit_cast(Words<D>, v[r | (1u << J)]);
-#pragma unroll
-            for (uint32_t d = 0; d < D; ++d) xchg<I>(x.w[d], y.w[d], upper);
-            v[r] = __builtin_bit_cast(V, x);
-            v[r | (1u << J)] = __builtin_bit_cast(V, y);
-        }
+    for (uint32_t r = 0; r < E; ++r) {
+        if (r & (1u << J)) continue;
+        xchg<I>(v[r].x, v[r | (1u << J)].x, upper);
+        xchg<I>(v[r].y, v[r | (1u << J)].y, upper);
     }
 }
 
 // Register bit J1 <-> register bit J2 (a renaming of the amplitudes).
 template <typename V, uint32_t E, uint32_t J1, uint32_t J2>
 __device__ __forceinline__ void reg_swap(V* v) {
-    if constexpr ((1u << J2) < E) {
 #pragma unroll
-        for (uint32_t r = 0; r < E; ++r) {
-            if (!((r >> J1) & 1u) || ((r >> J2) & 1u)) continue;
-            const uint32_t s = r ^ (1u << J1) ^ (1u << J2);
-            const V t = v[r];
-            v[r] = v[s];
-            v[s] = t;
-        }
+    for (uint32_t r = 0; r < E; ++r) {
+        if (!((r >> J1) & 1u) || ((r >> J2) & 1u)) continue;
+        const uint32_t s = r ^ (1u << J1) ^ (1u << J2);
+        const V t = v[r];
+        v[r] = v[s];
+        v[s] = t;
     }
 }
 
-// One 1-qubit block on register bit J.
+// One 1-qubit block on register bit J. Per pair every product is formed
+// before any sum, so the outputs are defined after the last read of the
+// inputs and take over their registers.
 template <uint32_t E, uint32_t J, bool kRe>
 __device__ __forceinline__ void dense1(double2* v, const double2* m) {
-    if constexpr ((1u << J) < E) {
-        const double2 m00 = m[0], m01 = m[1], m10 = m[2], m11 = m[3];
+    const double2 m00 = m[0], m01 = m[1], m10 = m[2], m11 = m[3];
 #pragma unroll
-        for (uint32_t r = 0; r < E; ++r) {
-            if (r & (1u << J)) continue;
-            const double2 a0 = v[r], a1 = v[r | (1u << J)];
-            double s0r, s0i;
-            if constexpr (kRe) {
-                s0r = m00.x * a0.x;
-                s0i = m00.x * a0.y;
-            } else {
-                s0r = m00.x * a0.x - m00.y * a0.y;
-                s0i = m00.x * a0.y + m00.y * a0.x;
-            }
-            s0r = s0r + (m01.x * a1.x - m01.y * a1.y);
-            s0i = s0i + (m01.x * a1.y + m01.y * a1.x);
-            const double s1r = (m10.x * a0.x - m10.y * a0.y) + (m11.x * a1.x - m11.y * a1.y);
-            const double s1i = (m10.x * a0.y + m10.y * a0.x) + (m11.x * a1.y + m11.y * a1.x);
-            v[r] = make_double2(s0r, s0i);
-            v[r | (1u << J)] = make_double2(s1r, s1i);
-        }
+    for (uint32_t r = 0; r < E; ++r) {
+        if (r & (1u << J)) continue;
+        const double2 a0 = v[r], a1 = v[r | (1u << J)];
+        const double p0 = m00.x * a0.x, p2 = m00.x * a0.y;
+        const double p1 = kRe ? 0.0 : m00.y * a0.y, p3 = kRe ? 0.0 : m00.y * a0.x;
+        const double p4 = m01.x * a1.x, p5 = m01.y * a1.y, p6 = m01.x * a1.y, p7 = m01.y * a1.x;
+        const double p8 = m10.x * a0.x, p9 = m10.y * a0.y, pa = m10.x * a0.y, pb = m10.y * a0.x;
+        const double pc = m11.x * a1.x, pd = m11.y * a1.y, pe = m11.x * a1.y, pf = m11.y * a1.x;
+        const double s0r = (kRe ? p0 : p0 - p1) + (p4 - p5);
+        const double s0i = (kRe ? p2 : p2 + p3) + (p6 + p7);
+        const double s1r = (p8 - p9) + (pc - pd);
+        const double s1i = (pa + pb) + (pe + pf);
+        v[r] = make_double2(s0r, s0i);
+        v[r | (1u << J)] = make_double2(s1r, s1i);
     }
 }
 
@@ -211,76 +238,69 @@ __device__ __forceinline__ void cmac(float& sr, float& si, float2 m, float2 a) {
 
 template <uint32_t E, uint32_t J, bool kRe>
 __device__ __forceinline__ void dense1(float2* v, const float2* m) {
-    if constexpr ((1u << J) < E) {
-        const float2 m00 = m[0], m01 = m[1], m10 = m[2], m11 = m[3];
+    const float2 m00 = m[0], m01 = m[1], m10 = m[2], m11 = m[3];
 #pragma unroll
-        for (uint32_t r = 0; r < E; ++r) {
-            if (r & (1u << J)) continue;
-            const float2 a0 = v[r], a1 = v[r | (1u << J)];
-            float s0r = m00.x * a0.x, s0i = m00.x * a0.y;
-            if constexpr (!kRe) {
-                s0r = __builtin_fmaf(-m00.y, a0.y, s0r);
-                s0i = __builtin_fmaf(m00.y, a0.x, s0i);
+    for (uint32_t r = 0; r < E; ++r) {
+        if (r & (1u << J)) continue;
+        const float2 a0 = v[r], a1 = v[r | (1u << J)];
+        float s0r = m00.x * a0.x, s0i = m00.x * a0.y;
+        if constexpr (!kRe) {
+            s0r = __builtin_fmaf(-m00.y, a0.y, s0r);
+            s0i = __builtin_fmaf(m00.y, a0.x, s0i);
+        }
+        cmac(s0r, s0i, m01, a1);
+        float s1r = 0.0f, s1i = 0.0f;
+        cmac(s1r, s1i, m10, a0);
+        cmac(s1r, s1i, m11, a1);
+        v[r] = make_float2(s0r, s0i);
+        v[r | (1u << J)] = make_float2(s1r, s1i);
+    }
+}
+
+// A 2-qubit block on register bits 0 (operand 0) and 1 (operand 1).
+template <uint32_t E>
+__device__ __forceinline__ void dense2(double2* v, const double2* m) {
+#pragma unroll
+    for (uint32_t hi = 0; hi < (E >> 2); ++hi) {
+        double2 a[4];
+#pragma unroll
+        for (uint32_t in = 0; in < 4; ++in) a[in] = v[in | (hi << 2)];
+#pragma unroll
+        for (uint32_t o = 0; o < 4; ++o) {
+            const double2 e0 = m[o * 4];
+            double sr = e0.x * a[0].x - e0.y * a[0].y;
+            double si = e0.x * a[0].y + e0.y * a[0].x;
+#pragma unroll
+            for (uint32_t in = 1; in < 4; ++in) {
+                const double2 e = m[o * 4 + in];
+                sr = sr + (e.x * a[in].x - e.y * a[in].y);
+                si = si + (e.x * a[in].y + e.y * a[in].x);
             }
-            cmac(s0r, s0i, m01, a1);
-            float s1r = 0.0f, s1i = 0.0f;
-            cmac(s1r, s1i, m10, a0);
-            cmac(s1r, s1i, m11, a1);
-            v[r] = make_float2(s0r, s0i);
-            v[r | (1u << J)] = make_float2(s1r, s1i);
+            v[o | (hi << 2)] = make_double2(sr, si);
         }
     }
 }
 
-// A k-qubit block whose operands are register bits 0..K-1 in operand order.
-template <uint32_t E, uint32_t K>
-__device__ __forceinline__ void densek(double2* v, const double2* m) {
-    if constexpr ((1u << K) <= E) {
-        constexpr uint32_t dim = 1u << K;
+template <uint32_t E>
+__device__ __forceinline__ void dense2(float2* v, const float2* m) {
 #pragma unroll
-        for (uint32_t hi = 0; hi < (E >> K); ++hi) {
-            double2 a[dim];
+    for (uint32_t hi = 0; hi < (E >> 2); ++hi) {
+        float2 a[4];
 #pragma unroll
-            for (uint32_t in = 0; in < dim; ++in) a[in] = v[in | (hi << K)];
+        for (uint32_t in = 0; in < 4; ++in) a[in] = v[in | (hi << 2)];
 #pragma unroll
-            for (uint32_t o = 0; o < dim; ++o) {
-                const double2 e0 = m[o * dim];
-                double sr = e0.x * a[0].x - e0.y * a[0].y;
-                double si = e0.x * a[0].y + e0.y * a[0].x;
+        for (uint32_t o = 0; o < 4; ++o) {
+            float sr = 0.0f, si = 0.0f;
 #pragma unroll
-                for (uint32_t in = 1; in < dim; ++in) {
-                    const double2 e = m[o * dim + in];
-                    sr = sr + (e.x * a[in].x - e.y * a[in].y);
-                    si = si + (e.x * a[in].y + e.y * a[in].x);
-                }
-                v[o | (hi << K)] = make_double2(sr, si);
-            }
-        }
-    }
-}
-
-template <uint32_t E, uint32_t K>
-__device__ __forceinline__ void densek(float2* v, const float2* m) {
-    if constexpr ((1u << K) <= E) {
-        constexpr uint32_t dim = 1u << K;
-#pragma unroll
-        for (uint32_t hi = 0; hi < (E >> K); ++hi) {
-            float2 a[dim];
-#pragma unroll
-            for (uint32_t in = 0; in < dim; ++in) a[in] = v[in | (hi << K)];
-#pragma unroll
-            for (uint32_t o = 0; o < dim; ++o) {
-                float sr = 0.0f, si = 0.0f;
-#pragma unroll
-                for (uint32_t in = 0; in < dim; ++in) cmac(sr, si, m[o * dim + in], a[in]);
-                v[o | (hi << K)] = make_float2(sr, si);
-            }
+            for (uint32_t in = 0; in < 4; ++in) cmac(sr, si, m[o * 4 + in], a[in]);
+            v[o | (hi << 2)] = make_float2(sr, si);
         }
     }
 }
 
 __device__ __forceinline__ double2 diag_mul(double2 d, double2 a) {
-    return make_double2(d.x * a.x - d.y * a.y, d.x * a.y + d.y * a.x);
+    const double pr = d.x * a.x, qr = d.y * a.y, pi = d.x * a.y, qi = d.y * a.x;
+    return make_double2(pr - qr, pi + qi);
 }
 
 __device__ __forceinline__ float2 diag_mul(float2 d, float2 a) {
@@ -310,14 +330,48 @@ __device__ __forceinline__ uint32_t diag_base(const WOp& op, uint64_t obase, uin
     return t;
 }
 
-#define MH_CASES_J(BASE, CALL) \
-    case BASE + 0: CALL(0); break; \
-    case BASE + 1: CALL(1); break; \
-    case BASE + 2: CALL(2); break; \
-    case BASE + 3: CALL(3); break; \
-    case BASE + 4: CALL(4); break;
+template <typename V, uint32_t E>
+__device__ __forceinline__ void apply_sign(V* v, const WOp& op, uint64_t obase, uint32_t lane) {
+    const uint64_t sh = op.neg_mask >> diag_base(op, obase, lane);
+#pragma unroll
+    for (uint32_t r = 0; r < E; ++r) flip_sign(v[r], uint32_t(sh >> op.rpart[r]) & 1u);
+}
 
-// Three waves per SIMD: each wave holds its window in about 130 VGPRs.
+template <typename V, uint32_t E>
+__device__ __forceinline__ void apply_diag(V* v, const WOp& op, const V* mats, uint64_t obase, uint32_t lane) {
+    const V* d = mats + op.mat;
+    const uint32_t t = diag_base(op, obase, lane);
+    // Entries gathered four at a time (bounds the registers they hold).
+#pragma unroll
+    for (uint32_t r = 0; r < E; ++r) {
+        v[r] = diag_mul(d[t | op.rpart[r]], v[r]);
+        if ((r & 3u) == 3u) __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// The stages of register bit J (exchanges into J, then its 1-qubit blocks).
+template <typename V, uint32_t E, uint32_t J>
+__device__ __forceinline__ void group_stages(uint64_t c, V* v, const WOp*& op, const V* mats, uint32_t lane) {
+    if constexpr ((1u << J) < E) {
+        if (!(c & (0xFFull << Stages::group(J)))) return;
+        if (c & (1ull << Stages::lane_swap(0, J))) lane_swap<V, E, 0, J>(v, lane);
+        if (c & (1ull << Stages::lane_swap(1, J))) lane_swap<V, E, 1, J>(v, lane);
+        if (c & (1ull << Stages::lane_swap(2, J))) lane_swap<V, E, 2, J>(v, lane);
+        if (c & (1ull << Stages::lane_swap(3, J))) lane_swap<V, E, 3, J>(v, lane);
+        if (c & (1ull << Stages::lane_swap(4, J))) lane_swap<V, E, 4, J>(v, lane);
+        if (c & (1ull << Stages::lane_swap(5, J))) lane_swap<V, E, 5, J>(v, lane);
+        if (c & (1ull << Stages::dense1_re(J))) dense1<E, J, true>(v, mats + (op++)->mat);
+        if (c & (1ull << Stages::dense1(J))) dense1<E, J, false>(v, mats + (op++)->mat);
+    }
+}
+
+template <typename V, uint32_t E, uint32_t RB, uint32_t J1, uint32_t J2>
+__device__ __forceinline__ void reg_swap_stage(uint64_t c, V* v) {
+    if constexpr (J2 < RB)
+        if (c & (1ull << Stages::reg_swap(J1, J2, RB))) reg_swap<V, E, J1, J2>(v);
+}
+
+// Three waves per SIMD: each wave holds its window in about 100 VGPRs.
 template <typename R>
 __global__ void __launch_bounds__(kWavesPerGroup * 64, 3) window_kernel(WPassArgs a) {
     using V = typename Shape<R>::V;
@@ -345,62 +399,30 @@ __global__ void __launch_bounds__(kWavesPerGroup * 64, 3) window_kernel(WPassArg
         }
     }
 
-    for (uint32_t i = 0; i < a.num_ops; ++i) {
-        const WOp& op = a.ops[i];
-        const V* m = mats + op.mat;
-        switch (op.code) {
-#define MH_D1(J) dense1<E, J, false>(v, m)
-            MH_CASES_J(kCodeDense1, MH_D1)
-#undef MH_D1
-#define MH_D1R(J) dense1<E, J, true>(v, m)
-            MH_CASES_J(kCodeDense1Re, MH_D1R)
-#undef MH_D1R
-            case kCodeDenseK + 2: densek<E, 2>(v, m); break;
-            case kCodeDiag: {
-                // Entries gathered four at a time (bounds the registers they hold).
-                const uint32_t t = diag_base(op, obase, lane);
-#pragma unroll
-                for (uint32_t r = 0; r < E; ++r) {
-                    v[r] = diag_mul(m[t | op.rpart[r]], v[r]);
-                    if ((r & 3u) == 3u) __builtin_amdgcn_sched_barrier(0);
-                }
-                break;
-            }
-            case kCodeDiagSign: {
-                const uint64_t sh = op.neg_mask >> diag_base(op, obase, lane);
-#pragma unroll
-                for (uint32_t r = 0; r < E; ++r) flip_sign(v[r], uint32_t(sh >> op.rpart[r]) & 1u);
-                break;
-            }
-#define MH_LS0(J) lane_swap<V, E, 0, J>(v, lane)
-            MH_CASES_J(kCodeLaneSwap + 0, MH_LS0)
-#undef MH_LS0
-#define MH_LS1(J) lane_swap<V, E, 1, J>(v, lane)
-            MH_CASES_J(kCodeLaneSwap + 8, MH_LS1)
-#undef MH_LS1
-#define MH_LS2(J) lane_swap<V, E, 2, J>(v, lane)
-            MH_CASES_J(kCodeLaneSwap + 16, MH_LS2)
-#undef MH_LS2
-#define MH_LS3(J) lane_swap<V, E, 3, J>(v, lane)
-            MH_CASES_J(kCodeLaneSwap + 24, MH_LS3)
-#undef MH_LS3
-#define MH_LS4(J) lane_swap<V, E, 4, J>(v, lane)
-            MH_CASES_J(kCodeLaneSwap + 32, MH_LS4)
-#undef MH_LS4
-#define MH_LS5(J) lane_swap<V, E, 5, J>(v, lane)
-            MH_CASES_J(kCodeLaneSwap + 40, MH_LS5)
-#undef MH_LS5
-            case kCodeRegSwap + 8 * 0 + 1: reg_swap<V, E, 0, 1>(v); break;
-            case kCodeRegSwap + 8 * 0 + 2: reg_swap<V, E, 0, 2>(v); break;
-            case kCodeRegSwap + 8 * 0 + 3: reg_swap<V, E, 0, 3>(v); break;
-            case kCodeRegSwap + 8 * 0 + 4: reg_swap<V, E, 0, 4>(v); break;
-            case kCodeRegSwap + 8 * 1 + 2: reg_swap<V, E, 1, 2>(v); break;
-            case kCodeRegSwap + 8 * 1 + 3: reg_swap<V, E, 1, 3>(v); break;
-            case kCodeRegSwap + 8 * 1 + 4: reg_swap<V, E, 1, 4>(v); break;
-            case kCodeRegSwap + 8 * 2 + 3: reg_swap<V, E, 2, 3>(v); break;
-            case kCodeRegSwap + 8 * 2 + 4: reg_swap<V, E, 2, 4>(v); break;
-            case kCodeRegSwap + 8 * 3 + 4: reg_swap<V, E, 3, 4>(v); break;
-            default: break;
+    const WOp* op = a.ops;
+    for (uint32_t b = 0; b < a.num_bundles; ++b) {
+        const uint64_t c = a.bundles[b];
+        if (c & (1ull << Stages::kSignEarly)) apply_sign<V, E>(v, *op++, obase, lane);
+        if (c & (1ull << Stages::kDiagEarly)) apply_diag<V, E>(v, *op++, mats, obase, lane);
+        group_stages<V, E, 0>(c, v, op, mats, lane);
+        group_stages<V, E, 1>(c, v, op, mats, lane);
+        group_stages<V, E, 2>(c, v, op, mats, lane);
+        group_stages<V, E, 3>(c, v, op, mats, lane);
+        group_stages<V, E, 4>(c, v, op, mats, lane);
+        if (c & (1ull << Stages::dense2(RB))) dense2<E>(v, mats + (op++)->mat);
+        if (c & (1ull << Stages::sign_late(RB))) apply_sign<V, E>(v, *op++, obase, lane);
+        if (c & (1ull << Stages::diag_late(RB))) apply_diag<V, E>(v, *op++, mats, obase, lane);
+        if (c >> Stages::reg_swap(0, 1, RB)) {
+            reg_swap_stage<V, E, RB, 0, 1>(c, v);
+            reg_swap_stage<V, E, RB, 0, 2>(c, v);
+            reg_swap_stage<V, E, RB, 0, 3>(c, v);
+            reg_swap_stage<V, E, RB, 0, 4>(c, v);
+            reg_swap_stage<V, E, RB, 1, 2>(c, v);
+            reg_swap_stage<V, E, RB, 1, 3>(c, v);
+            reg_swap_stage<V, E, RB, 1, 4>(c, v);
+            reg_swap_stage<V, E, RB, 2, 3>(c, v);
+            reg_swap_stage<V, E, RB, 2, 4>(c, v);
+            reg_swap_stage<V, E, RB, 3, 4>(c, v);
         }
     }
 
@@ -430,8 +452,6 @@ __global__ void __launch_bounds__(kWavesPerGroup * 64, 3) window_kernel(WPassArg
         }
     }
 }
-
-#undef MH_CASES_J
 
 // ---- host: passes
 
@@ -473,11 +493,14 @@ public:
     using V = typename Shape<R>::V;
     static constexpr uint32_t RB = Shape<R>::kRegBits, L = Shape<R>::kLow, W = width_of<R>(), E = 1u << RB;
 
-    PassCompiler(std::vector<WOp>& ops, std::vector<V>& mats) : ops_(ops), mats_(mats) {}
+    PassCompiler(std::vector<uint64_t>& bundles, std::vector<WOp>& ops, std::vector<V>& mats)
+        : bundles_(bundles), ops_(ops), mats_(mats) {}
 
     WPassArgs compile(uint32_t n, const std::vector<Block>& blocks, const WindowPass& pass, bool normalize) {
         WPassArgs a;
         std::memset(&a, 0, sizeof(a));
+        cur_ = 0;
+        last_ = -1;
         // Window qubits: the low run, the dense operands, then the lowest others.
         uint64_t wmask = pass.dense_mask | ((1ull << L) - 1ull);
         for (uint32_t q = 0; q < n && static_cast<uint32_t>(__builtin_popcountll(wmask)) < W; ++q) wmask |= 1ull << q;
@@ -532,7 +555,7 @@ public:
                     else
                         lane_swap(static_cast<uint32_t>(lane_index(p)), t);
                 }
-                emit_densek(b);
+                emit_dense2(b);
             }
         }
         // The low qubits back on the low lane bits for the store.
@@ -551,6 +574,7 @@ public:
             }
             lane_swap(l, static_cast<uint32_t>(j));
         }
+        flush();
         for (uint32_t i = 0; i < kLaneBits; ++i) a.store_lane_q[i] = wq_[lane_[i]];
         for (uint32_t j = 0; j < RB; ++j) a.store_reg_q[j] = wq_[reg_[j]];
         a.normalize = normalize ? 1u : 0u;
@@ -631,21 +655,35 @@ private:
         return best;
     }
 
-    void push_op(const WOp& op) { ops_.push_back(op); }
+    // Appends a stage to the open bundle when one of its positions comes after
+    // the bundle's last stage, else starts a new bundle.
+    void put(uint32_t first, int second, const WOp* rec) {
+        uint32_t at = first;
+        if (static_cast<int>(first) <= last_) {
+            if (second > last_) {
+                at = static_cast<uint32_t>(second);
+            } else {
+                flush();
+            }
+        }
+        cur_ |= 1ull << at;
+        last_ = static_cast<int>(at);
+        if (rec) ops_.push_back(*rec);
+    }
+
+    void flush() {
+        if (cur_) bundles_.push_back(cur_);
+        cur_ = 0;
+        last_ = -1;
+    }
 
     void lane_swap(uint32_t i, uint32_t j) {
-        WOp op;
-        std::memset(&op, 0, sizeof(op));
-        op.code = kCodeLaneSwap + 8 * i + j;
-        push_op(op);
+        put(Stages::lane_swap(i, j), -1, nullptr);
         std::swap(lane_[i], reg_[j]);
     }
 
     void reg_swap(uint32_t j1, uint32_t j2) {
-        WOp op;
-        std::memset(&op, 0, sizeof(op));
-        op.code = kCodeRegSwap + 8 * std::min(j1, j2) + std::max(j1, j2);
-        push_op(op);
+        put(Stages::reg_swap(std::min(j1, j2), std::max(j1, j2), RB), -1, nullptr);
         std::swap(reg_[j1], reg_[j2]);
     }
 
@@ -663,17 +701,15 @@ private:
     void emit_dense1(const Block& b, uint32_t j) {
         WOp op;
         std::memset(&op, 0, sizeof(op));
-        op.code = (b.matrix[0].imag() == 0.0 ? kCodeDense1Re : kCodeDense1) + j;
         op.mat = push_matrix(b.matrix);
-        push_op(op);
+        put(b.matrix[0].imag() == 0.0 ? Stages::dense1_re(j) : Stages::dense1(j), -1, &op);
     }
 
-    void emit_densek(const Block& b) {
+    void emit_dense2(const Block& b) {
         WOp op;
         std::memset(&op, 0, sizeof(op));
-        op.code = kCodeDenseK + static_cast<uint32_t>(b.operands.size());
         op.mat = push_matrix(b.matrix);
-        push_op(op);
+        put(Stages::dense2(RB), -1, &op);
     }
 
     void emit_diag(const Block& b) {
@@ -681,7 +717,6 @@ private:
         const std::size_t dim = std::size_t{1} << k;
         WOp op;
         std::memset(&op, 0, sizeof(op));
-        op.code = sign_diagonal(b) ? kCodeDiagSign : kCodeDiag;
         std::memset(op.lane_bit, kNone, sizeof(op.lane_bit));
         for (uint32_t t = 0; t < k; ++t) {
             const uint32_t q = b.operands[t];
@@ -707,11 +742,17 @@ private:
             if (d[e].real() == -1.0) op.neg_mask |= 1ull << e;
         }
         op.mat = push_matrix(d);
-        push_op(op);
+        if (sign_diagonal(b))
+            put(Stages::kSignEarly, static_cast<int>(Stages::sign_late(RB)), &op);
+        else
+            put(Stages::kDiagEarly, static_cast<int>(Stages::diag_late(RB)), &op);
     }
 
+    std::vector<uint64_t>& bundles_;
     std::vector<WOp>& ops_;
     std::vector<V>& mats_;
+    uint64_t cur_ = 0;
+    int last_ = -1;
     std::vector<const Block*> blocks_;
     uint8_t wq_[16] = {};
     uint8_t pos_of_[64] = {};
@@ -726,19 +767,23 @@ void launch_window_passes(void* state, uint32_t n, const std::vector<Block>& blo
     constexpr uint32_t W = width_of<R>();
     Context& ctx = context();
     std::vector<WPassArgs> args(passes.size());
-    std::vector<uint32_t> op_begin(passes.size(), 0);
+    std::vector<uint32_t> bundle_begin(passes.size(), 0), op_begin(passes.size(), 0);
+    std::vector<uint64_t> bundles;
     std::vector<WOp> ops;
     std::vector<V> mats;
-    PassCompiler<R> compiler(ops, mats);
+    PassCompiler<R> compiler(bundles, ops, mats);
     for (std::size_t p = 0; p < passes.size(); ++p) {
+        bundle_begin[p] = static_cast<uint32_t>(bundles.size());
         op_begin[p] = static_cast<uint32_t>(ops.size());
         if (passes[p].generic) continue;
         const bool last = p + 1 == passes.size();
         args[p] = compiler.compile(n, blocks, passes[p], last && sizeof(R) == 8);
-        args[p].num_ops = static_cast<uint32_t>(ops.size()) - op_begin[p];
+        args[p].num_bundles = static_cast<uint32_t>(bundles.size()) - bundle_begin[p];
     }
+    DeviceBuffer dbundles(std::max<std::size_t>(bundles.size(), 1) * sizeof(uint64_t));
     DeviceBuffer dops(std::max<std::size_t>(ops.size(), 1) * sizeof(WOp));
     DeviceBuffer dmats(std::max<std::size_t>(mats.size(), 1) * sizeof(V));
+    upload(dbundles.get(), bundles.data(), bundles.size() * sizeof(uint64_t));
     upload(dops.get(), ops.data(), ops.size() * sizeof(WOp));
     upload(dmats.get(), mats.data(), mats.size() * sizeof(V));
 
@@ -756,6 +801,7 @@ void launch_window_passes(void* state, uint32_t n, const std::vector<Block>& blo
         } else {
             WPassArgs a = args[p];
             a.state = state;
+            a.bundles = dbundles.as<uint64_t>() + bundle_begin[p];
             a.ops = dops.as<WOp>() + op_begin[p];
             a.mats = dmats.get();
             if (n < W || n - W > 30) fail(MH_ERR_INTERNAL, "window pass shape out of range");
