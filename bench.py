@@ -62,9 +62,12 @@ def main() -> int:
     ap.add_argument("--fusion-width", type=int, default=0)
     ap.add_argument("--window", type=int, default=0)
     ap.add_argument("--max-pass-cost", type=int, default=0)
-    ap.add_argument("--cpu-qubits", type=int, default=25)
-    ap.add_argument("--cpu-layers", type=int, default=2)
+    ap.add_argument("--cpu-qubits", type=int, default=28)
+    ap.add_argument("--cpu-layers", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run the library's calibration (its streaming copy of 2 GiB each way is the "
+                         "reference for scaling HBM traffic counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -75,6 +78,9 @@ def main() -> int:
     from menos_amd import hip as H
     from menos_amd import workloads as W
 
+    if args.copy_probe:
+        cal = H.calibrate()
+        log(f"calibration: copy {cal.hbm_bytes_per_second / 1e9:.0f} GB/s")
     info = H.device_info()
     log(f"device {info.name.decode()} {info.arch.decode()} CUs={info.compute_units} "
         f"HBM={info.total_bytes / 2**30:.0f} GiB free={info.free_bytes / 2**30:.0f} GiB")

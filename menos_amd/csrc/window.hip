@@ -45,7 +45,6 @@ namespace {
 constexpr uint32_t kLaneBits = 6;
 constexpr uint32_t kWavesPerGroup = 4;
 constexpr uint32_t kNone = 255;
-constexpr uint32_t kMaxRuns = 16;
 constexpr uint32_t kMaxWindowDense = 2;  // widest dense block a window pass applies
 
 template <typename R> struct Shape;
@@ -89,10 +88,12 @@ struct Stages {
 static_assert(Stages::reg_swap(3, 4, 5) < 64, "stages of one bundle fit 64 bits");
 
 // The parameters of a stage that has them (blocks, not exchanges), in stage
-// order within a bundle.
+// order within a bundle. A sign stage applies a run of consecutive sign
+// diagonals at once: sign flips commute and compose exactly, so the run is
+// one XOR mask over the lane's amplitudes.
 struct WOp {
-    uint32_t mat;            // first entry in the matrix table
-    uint32_t num_outer;      // diagonal: operands outside the window
+    uint32_t mat;            // first entry in the matrix table; sign stage: first term
+    uint32_t num_outer;      // diagonal: operands outside the window; sign stage: terms
     uint64_t neg_mask;       // sign diagonal: bit e set when entry e is -1
     uint8_t lane_bit[8];     // diagonal: operand bit that lane bit i supplies, or kNone
     uint8_t outer_q[8];      // diagonal: global qubit of the m-th operand outside the window
@@ -100,16 +101,26 @@ struct WOp {
     uint8_t rpart[32];       // diagonal: operand bits that register index r supplies
 };
 
+// One sign diagonal of a sign stage: where each operand's bit comes from.
+struct SignTerm {
+    uint64_t neg_mask;       // bit e set when entry e is -1
+    uint8_t k;
+    uint8_t src[6];          // operand u: lane bit 0..5, register bit 8 + j, or 16 (outside the window)
+    uint8_t q[6];            // outside the window: the global qubit
+    uint8_t pad[3];
+};
+
 struct WPassArgs {
     void* state;
     const uint64_t* bundles;
     const WOp* ops;
+    const SignTerm* terms;
     const void* mats;
     uint32_t num_bundles;
     uint32_t normalize;       // add +0 to every component before the store
-    uint32_t num_runs;        // runs of consecutive qubits outside the window
+    uint32_t num_outer;       // qubits outside the window
     uint32_t pad;
-    uint8_t run_start[kMaxRuns], run_len[kMaxRuns];
+    uint8_t outer_q[32];      // wave index bit b -> global qubit outer_q[b]
     uint8_t load_lane_q[8], load_reg_q[8];    // global qubit of each lane / register bit at the load
     uint8_t store_lane_q[8], store_reg_q[8];  // ... and at the store
 };
@@ -117,13 +128,37 @@ struct WPassArgs {
 // The qubits outside the window, taken from the wave index (a bit deposit).
 __device__ __forceinline__ uint64_t outer_base(uint32_t wave, const WPassArgs& a) {
     uint64_t base = 0;
-    uint32_t src = 0;
-    for (uint32_t i = 0; i < a.num_runs; ++i) {
-        const uint32_t len = a.run_len[i];
-        base |= (uint64_t(wave >> src) & ((1ull << len) - 1ull)) << a.run_start[i];
-        src += len;
-    }
+    for (uint32_t b = 0; b < a.num_outer; ++b) base |= uint64_t((wave >> b) & 1u) << a.outer_q[b];
     return base;
+}
+
+// The state streams through once per pass: non-temporal loads and stores
+// keep it from displacing the op tables and the other waves' lines in the caches.
+typedef double VD2 __attribute__((ext_vector_type(2)));
+typedef float VF2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ double2 load_amp(const double2* p) {
+    const VD2 x = __builtin_nontemporal_load(reinterpret_cast<const VD2*>(p));
+    return make_double2(x[0], x[1]);
+}
+
+__device__ __forceinline__ float2 load_amp(const float2* p) {
+    const VF2 x = __builtin_nontemporal_load(reinterpret_cast<const VF2*>(p));
+    return make_float2(x[0], x[1]);
+}
+
+__device__ __forceinline__ void store_amp(double2* p, double2 v) {
+    VD2 x;
+    x[0] = v.x;
+    x[1] = v.y;
+    __builtin_nontemporal_store(x, reinterpret_cast<VD2*>(p));
+}
+
+__device__ __forceinline__ void store_amp(float2* p, float2 v) {
+    VF2 x;
+    x[0] = v.x;
+    x[1] = v.y;
+    __builtin_nontemporal_store(x, reinterpret_cast<VF2*>(p));
 }
 
 __device__ __forceinline__ uint64_t lane_offset(uint32_t lane, const uint8_t* q) {
@@ -330,11 +365,44 @@ __device__ __forceinline__ uint32_t diag_base(const WOp& op, uint64_t obase, uin
     return t;
 }
 
+// Register index bits of register bit j (bit r set when bit j of r is).
+__constant__ const uint32_t kRegPattern[5] = {0xAAAAAAAAu, 0xCCCCCCCCu, 0xF0F0F0F0u, 0xFF00FF00u, 0xFFFF0000u};
+
+// A run of sign diagonals: bit r of m tells whether amplitude r changes sign.
+// Each term adds (XOR) the indicator of its -1 entries, a sum of minterms
+// over its operand bits; an operand bit as a mask over r is a constant for a
+// register bit and all-or-nothing for a lane bit or a qubit outside the window.
 template <typename V, uint32_t E>
-__device__ __forceinline__ void apply_sign(V* v, const WOp& op, uint64_t obase, uint32_t lane) {
-    const uint64_t sh = op.neg_mask >> diag_base(op, obase, lane);
+__device__ __forceinline__ void apply_signs(V* v, const WOp& op, const SignTerm* terms, uint64_t obase,
+                                            uint32_t lane) {
+    constexpr uint32_t full = E == 32 ? 0xFFFFFFFFu : (1u << E) - 1u;
+    uint32_t m = 0;
+    for (uint32_t t = 0; t < op.num_outer; ++t) {
+        const SignTerm& st = terms[op.mat + t];
+        uint32_t a[6];
 #pragma unroll
-    for (uint32_t r = 0; r < E; ++r) flip_sign(v[r], uint32_t(sh >> op.rpart[r]) & 1u);
+        for (uint32_t u = 0; u < 6; ++u) {
+            a[u] = 0;
+            if (u >= st.k) continue;
+            const uint32_t src = st.src[u];
+            if (src < 8)
+                a[u] = ((lane >> src) & 1u) ? full : 0u;
+            else if (src < 16)
+                a[u] = kRegPattern[src - 8] & full;
+            else
+                a[u] = ((obase >> st.q[u]) & 1ull) ? full : 0u;
+        }
+        for (uint64_t nm = st.neg_mask; nm != 0; nm &= nm - 1) {
+            const uint32_t e = static_cast<uint32_t>(__builtin_ctzll(nm));
+            uint32_t mt = full;
+#pragma unroll
+            for (uint32_t u = 0; u < 6; ++u)
+                if (u < st.k) mt &= ((e >> u) & 1u) ? a[u] : ~a[u];
+            m ^= mt;
+        }
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < E; ++r) flip_sign(v[r], (m >> r) & 1u);
 }
 
 template <typename V, uint32_t E>
@@ -371,9 +439,9 @@ __device__ __forceinline__ void reg_swap_stage(uint64_t c, V* v) {
         if (c & (1ull << Stages::reg_swap(J1, J2, RB))) reg_swap<V, E, J1, J2>(v);
 }
 
-// Three waves per SIMD: each wave holds its window in about 100 VGPRs.
+// Four waves per SIMD: each wave holds its window in at most 128 VGPRs.
 template <typename R>
-__global__ void __launch_bounds__(kWavesPerGroup * 64, 3) window_kernel(WPassArgs a) {
+__global__ void __launch_bounds__(kWavesPerGroup * 64, 4) window_kernel(WPassArgs a) {
     using V = typename Shape<R>::V;
     constexpr uint32_t RB = Shape<R>::kRegBits;
     constexpr uint32_t E = 1u << RB;
@@ -395,14 +463,14 @@ __global__ void __launch_bounds__(kWavesPerGroup * 64, 3) window_kernel(WPassArg
 #pragma unroll
             for (uint32_t j = 0; j < RB; ++j)
                 if ((r >> j) & 1u) h |= rq[j];
-            v[r] = src[h];
+            v[r] = load_amp(src + h);
         }
     }
 
     const WOp* op = a.ops;
     for (uint32_t b = 0; b < a.num_bundles; ++b) {
         const uint64_t c = a.bundles[b];
-        if (c & (1ull << Stages::kSignEarly)) apply_sign<V, E>(v, *op++, obase, lane);
+        if (c & (1ull << Stages::kSignEarly)) apply_signs<V, E>(v, *op++, a.terms, obase, lane);
         if (c & (1ull << Stages::kDiagEarly)) apply_diag<V, E>(v, *op++, mats, obase, lane);
         group_stages<V, E, 0>(c, v, op, mats, lane);
         group_stages<V, E, 1>(c, v, op, mats, lane);
@@ -410,7 +478,7 @@ __global__ void __launch_bounds__(kWavesPerGroup * 64, 3) window_kernel(WPassArg
         group_stages<V, E, 3>(c, v, op, mats, lane);
         group_stages<V, E, 4>(c, v, op, mats, lane);
         if (c & (1ull << Stages::dense2(RB))) dense2<E>(v, mats + (op++)->mat);
-        if (c & (1ull << Stages::sign_late(RB))) apply_sign<V, E>(v, *op++, obase, lane);
+        if (c & (1ull << Stages::sign_late(RB))) apply_signs<V, E>(v, *op++, a.terms, obase, lane);
         if (c & (1ull << Stages::diag_late(RB))) apply_diag<V, E>(v, *op++, mats, obase, lane);
         if (c >> Stages::reg_swap(0, 1, RB)) {
             reg_swap_stage<V, E, RB, 0, 1>(c, v);
@@ -448,7 +516,7 @@ __global__ void __launch_bounds__(kWavesPerGroup * 64, 3) window_kernel(WPassArg
 #pragma unroll
             for (uint32_t j = 0; j < RB; ++j)
                 if ((r >> j) & 1u) h |= rq[j];
-            dst[h] = v[r];
+            store_amp(dst + h, v[r]);
         }
     }
 }
@@ -493,14 +561,16 @@ public:
     using V = typename Shape<R>::V;
     static constexpr uint32_t RB = Shape<R>::kRegBits, L = Shape<R>::kLow, W = width_of<R>(), E = 1u << RB;
 
-    PassCompiler(std::vector<uint64_t>& bundles, std::vector<WOp>& ops, std::vector<V>& mats)
-        : bundles_(bundles), ops_(ops), mats_(mats) {}
+    PassCompiler(std::vector<uint64_t>& bundles, std::vector<WOp>& ops, std::vector<SignTerm>& terms,
+                 std::vector<V>& mats)
+        : bundles_(bundles), ops_(ops), terms_(terms), mats_(mats) {}
 
     WPassArgs compile(uint32_t n, const std::vector<Block>& blocks, const WindowPass& pass, bool normalize) {
         WPassArgs a;
         std::memset(&a, 0, sizeof(a));
         cur_ = 0;
         last_ = -1;
+        open_sign_ = -1;
         // Window qubits: the low run, the dense operands, then the lowest others.
         uint64_t wmask = pass.dense_mask | ((1ull << L) - 1ull);
         for (uint32_t q = 0; q < n && static_cast<uint32_t>(__builtin_popcountll(wmask)) < W; ++q) wmask |= 1ull << q;
@@ -512,17 +582,20 @@ public:
                 wq_[count] = static_cast<uint8_t>(q);
                 pos_of_[q] = static_cast<uint8_t>(count++);
             }
-        for (uint32_t q = 0; q < n;) {
-            if ((wmask >> q) & 1ull) {
-                ++q;
-                continue;
-            }
-            const uint32_t s = q;
-            while (q < n && !((wmask >> q) & 1ull)) ++q;
-            if (a.num_runs >= kMaxRuns) fail(MH_ERR_INTERNAL, "window pass shape");
-            a.run_start[a.num_runs] = static_cast<uint8_t>(s);
-            a.run_len[a.num_runs] = static_cast<uint8_t>(q - s);
-            ++a.num_runs;
+        // Wave index bits -> outside qubits: the lowest three first (eight
+        // consecutive waves cover a contiguous kilobyte of every segment they
+        // touch), then the others from the highest down. Measured on MI355X
+        // at 33 qubits, this ordering holds a pass whose window qubits are
+        // all above qubit 22 to about 60 ms (about 93 ms with plain ascending
+        // order) and costs no more than a few percent on other windows.
+        {
+            uint8_t outside[64];
+            uint32_t count_out = 0;
+            for (uint32_t q = 0; q < n; ++q)
+                if (!((wmask >> q) & 1ull)) outside[count_out++] = static_cast<uint8_t>(q);
+            const uint32_t head = std::min<uint32_t>(3, count_out);
+            for (uint32_t k = 0; k < head; ++k) a.outer_q[a.num_outer++] = outside[k];
+            for (uint32_t k = count_out; k-- > head;) a.outer_q[a.num_outer++] = outside[k];
         }
         blocks_.clear();
         for (uint32_t b : pass.blocks) blocks_.push_back(&blocks[b]);
@@ -668,6 +741,7 @@ private:
         }
         cur_ |= 1ull << at;
         last_ = static_cast<int>(at);
+        open_sign_ = -1;
         if (rec) ops_.push_back(*rec);
     }
 
@@ -713,6 +787,10 @@ private:
     }
 
     void emit_diag(const Block& b) {
+        if (sign_diagonal(b)) {
+            emit_sign(b);
+            return;
+        }
         const uint32_t k = static_cast<uint32_t>(b.operands.size());
         const std::size_t dim = std::size_t{1} << k;
         WOp op;
@@ -737,22 +815,53 @@ private:
                 if ((r >> j) & 1u) op.rpart[r] |= static_cast<uint8_t>(1u << t);
         }
         std::vector<cplx> d(dim);
-        for (std::size_t e = 0; e < dim; ++e) {
-            d[e] = b.matrix[e * dim + e];
-            if (d[e].real() == -1.0) op.neg_mask |= 1ull << e;
-        }
+        for (std::size_t e = 0; e < dim; ++e) d[e] = b.matrix[e * dim + e];
         op.mat = push_matrix(d);
-        if (sign_diagonal(b))
-            put(Stages::kSignEarly, static_cast<int>(Stages::sign_late(RB)), &op);
-        else
-            put(Stages::kDiagEarly, static_cast<int>(Stages::diag_late(RB)), &op);
+        put(Stages::kDiagEarly, static_cast<int>(Stages::diag_late(RB)), &op);
+    }
+
+    // A sign diagonal joins the sign stage just emitted, if nothing came after it.
+    void emit_sign(const Block& b) {
+        const uint32_t k = static_cast<uint32_t>(b.operands.size());
+        const std::size_t dim = std::size_t{1} << k;
+        SignTerm t;
+        std::memset(&t, 0, sizeof(t));
+        t.k = static_cast<uint8_t>(k);
+        for (uint32_t u = 0; u < k; ++u) {
+            const uint32_t q = b.operands[u];
+            const uint8_t p = pos_of_[q];
+            if (p == kNone) {
+                t.src[u] = 16;
+                t.q[u] = static_cast<uint8_t>(q);
+            } else if (lane_index(p) >= 0) {
+                t.src[u] = static_cast<uint8_t>(lane_index(p));
+            } else {
+                t.src[u] = static_cast<uint8_t>(8 + reg_index(p));
+            }
+        }
+        for (std::size_t e = 0; e < dim; ++e)
+            if (b.matrix[e * dim + e].real() == -1.0) t.neg_mask |= 1ull << e;
+        if (open_sign_ >= 0) {
+            terms_.push_back(t);
+            ++ops_[static_cast<std::size_t>(open_sign_)].num_outer;
+            return;
+        }
+        WOp op;
+        std::memset(&op, 0, sizeof(op));
+        op.mat = static_cast<uint32_t>(terms_.size());
+        op.num_outer = 1;
+        terms_.push_back(t);
+        put(Stages::kSignEarly, static_cast<int>(Stages::sign_late(RB)), &op);
+        open_sign_ = static_cast<int>(ops_.size()) - 1;
     }
 
     std::vector<uint64_t>& bundles_;
     std::vector<WOp>& ops_;
+    std::vector<SignTerm>& terms_;
     std::vector<V>& mats_;
     uint64_t cur_ = 0;
     int last_ = -1;
+    int open_sign_ = -1;  // index in ops_ of a sign stage that may take more terms
     std::vector<const Block*> blocks_;
     uint8_t wq_[16] = {};
     uint8_t pos_of_[64] = {};
@@ -770,8 +879,9 @@ void launch_window_passes(void* state, uint32_t n, const std::vector<Block>& blo
     std::vector<uint32_t> bundle_begin(passes.size(), 0), op_begin(passes.size(), 0);
     std::vector<uint64_t> bundles;
     std::vector<WOp> ops;
+    std::vector<SignTerm> terms;
     std::vector<V> mats;
-    PassCompiler<R> compiler(bundles, ops, mats);
+    PassCompiler<R> compiler(bundles, ops, terms, mats);
     for (std::size_t p = 0; p < passes.size(); ++p) {
         bundle_begin[p] = static_cast<uint32_t>(bundles.size());
         op_begin[p] = static_cast<uint32_t>(ops.size());
@@ -783,6 +893,8 @@ void launch_window_passes(void* state, uint32_t n, const std::vector<Block>& blo
     DeviceBuffer dbundles(std::max<std::size_t>(bundles.size(), 1) * sizeof(uint64_t));
     DeviceBuffer dops(std::max<std::size_t>(ops.size(), 1) * sizeof(WOp));
     DeviceBuffer dmats(std::max<std::size_t>(mats.size(), 1) * sizeof(V));
+    DeviceBuffer dterms(std::max<std::size_t>(terms.size(), 1) * sizeof(SignTerm));
+    upload(dterms.get(), terms.data(), terms.size() * sizeof(SignTerm));
     upload(dbundles.get(), bundles.data(), bundles.size() * sizeof(uint64_t));
     upload(dops.get(), ops.data(), ops.size() * sizeof(WOp));
     upload(dmats.get(), mats.data(), mats.size() * sizeof(V));
@@ -803,6 +915,7 @@ void launch_window_passes(void* state, uint32_t n, const std::vector<Block>& blo
             a.state = state;
             a.bundles = dbundles.as<uint64_t>() + bundle_begin[p];
             a.ops = dops.as<WOp>() + op_begin[p];
+            a.terms = dterms.as<SignTerm>();
             a.mats = dmats.get();
             if (n < W || n - W > 30) fail(MH_ERR_INTERNAL, "window pass shape out of range");
             const uint32_t waves_log = n - W;

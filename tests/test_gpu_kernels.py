@@ -95,6 +95,38 @@ def test_signed_zero_matrices_and_states(orc, hip, n):
     assert got.tobytes() == want.tobytes()
 
 
+def test_runs_of_sign_diagonals(orc, hip):
+    """Consecutive sign diagonals (entries +-1) of 1-4 qubits on lane, register
+    and outside-window qubits, mixed with 1-qubit and phase diagonals: a pass
+    applies each run as one XOR mask."""
+    rng = np.random.default_rng(11)
+    for n in (10, 12, 13):
+        for trial in range(4):
+            blocks = []
+            for _ in range(40):
+                r = rng.random()
+                k = int(rng.integers(1, 5))
+                ops = rng.permutation(n)[:k].tolist()
+                if r < 0.6:
+                    m = np.diag(rng.choice([1.0, -1.0], 1 << k).astype(np.complex128))
+                elif r < 0.8:
+                    ops = ops[:1]
+                    m = random_unitary(1, rng)
+                else:
+                    m = np.diag(np.exp(1j * rng.uniform(-np.pi, np.pi, 1 << k)))
+                blocks.append((m, ops))
+            s = random_state(n, trial)
+            d = hip.DeviceState(n)
+            d.upload(s)
+            d.apply_blocks(blocks)
+            got = d.download()
+            d.free()
+            want = s.copy()
+            for m, ops in blocks:
+                orc.apply_generic(want, m, ops)
+            assert got.tobytes() == want.tobytes(), (n, trial)
+
+
 def test_block_list_equals_gate_by_gate(orc, hip):
     """Many blocks per LDS-window pass give the bits of one launch per block."""
     rng = np.random.default_rng(99)
