@@ -21,6 +21,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0  # MI355X datasheet
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "window_kernel_33q_c128_traffic.md")
+
+
+def measured_traffic(qubits: int, precision: str):
+    """HBM bytes per window-kernel launch from the committed PMC summary
+    (FETCH_SIZE + WRITE_SIZE scaled by the library's copy kernel), for the
+    33-qubit complex128 pass it was measured on; None for other shapes."""
+    if qubits != 33 or precision != "c128" or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    return None
 
 
 def log(msg: str) -> None:
@@ -146,7 +160,9 @@ def main() -> int:
             "max_pass_cost": args.max_pass_cost or "default",
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": measured_traffic(n, args.precision),
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/window_kernel_33q_c128_traffic.md",
                      "kernel": "window_kernel", "avg_launch_ms": round(avg_ms, 4),
                      "algorithmic_bytes_per_launch": pass_bytes},
     }
