@@ -64,6 +64,50 @@ def cpu_baseline(qubits: int, layers: int, threads: int) -> dict:
                       f"{secs:.2f} s on {threads} threads"}
 
 
+def batched(args, H, W) -> dict:
+    """The batched estimator workload: `args.batch_circuits` ansatz circuits
+    (n uniform on 10..22, depth 4..12, RY and RZ on every qubit then a CNOT
+    ladder per layer, 8 Pauli strings of weight 1..4 each), all executed by
+    mh_batch_execute at complex64 and complex128. The error is the largest
+    |<P>_c64 - <P>_c128| over every circuit and term. The per-circuit GPU path
+    (one mh_engine_execute per circuit) is timed on the first
+    `args.per_circuit_sample` circuits."""
+    import numpy as np
+    circuits, terms = W.batch_workload(args.batch_circuits, 2026)
+    warm = H.batch_execute(circuits[:64], terms[:64], precision=H.C64)
+    del warm
+    out = {"workload": f"{len(circuits)} ansatz circuits, n 10..22, depth 4..12, 8 Paulis each",
+           "circuits": len(circuits)}
+    res = {}
+    for name, prec in (("c64", H.C64), ("c128", H.C128)):
+        r = H.batch_execute(circuits, terms, precision=prec)
+        res[name] = r
+        lat = np.sort(r.completion_seconds) * 1e3
+        out[name] = {"circuits_per_s": round(len(circuits) / r.total_seconds, 1),
+                     "seconds": round(r.total_seconds, 4), "plan_seconds": round(r.plan_seconds, 4),
+                     "device_seconds": round(r.device_seconds, 4), "rounds": r.rounds, "launches": r.launches,
+                     "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                     "p99_ms": round(float(np.percentile(lat, 99)), 2)}
+        log(f"batched {name}: {out[name]}")
+    out["max_abs_err_c64_vs_c128"] = float(max(np.max(np.abs(a - b)) if a.size else 0.0
+                                               for a, b in zip(res["c64"].expectation, res["c128"].expectation)))
+    k = min(args.per_circuit_sample, len(circuits))
+    eng = H.HipDenseEngine()
+    eng.execute(circuits[0], precision=H.C64, terms=terms[0], statevector=False, targets=[], width=0)
+    per = {}
+    for name, prec in (("c64", H.C64), ("c128", H.C128)):
+        t0 = time.perf_counter()
+        for c, t in zip(circuits[:k], terms[:k]):
+            eng.execute(c, precision=prec, terms=t, statevector=False, targets=[], width=0)
+        per[name] = k / (time.perf_counter() - t0)
+    out["per_circuit_path"] = {"sample": k, "c64_circuits_per_s": round(per["c64"], 1),
+                               "c128_circuits_per_s": round(per["c128"], 1)}
+    out["speedup_c64"] = round(out["c64"]["circuits_per_s"] / per["c64"], 2)
+    out["speedup_c128"] = round(out["c128"]["circuits_per_s"] / per["c128"], 2)
+    log(f"per-circuit path on {k} circuits: {out['per_circuit_path']}; speedup c64 {out['speedup_c64']}x")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +123,9 @@ def main() -> int:
     ap.add_argument("--cpu-qubits", type=int, default=28)
     ap.add_argument("--cpu-layers", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-circuits", type=int, default=4096)
+    ap.add_argument("--per-circuit-sample", type=int, default=256)
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run the library's calibration (its streaming copy of 2 GiB each way is the "
                          "reference for scaling HBM traffic counters)")
@@ -166,6 +213,8 @@ def main() -> int:
                      "kernel": "window_kernel", "avg_launch_ms": round(avg_ms, 4),
                      "algorithmic_bytes_per_launch": pass_bytes},
     }
+    if not args.no_batch:
+        out["batched"] = batched(args, H, W)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline: {args.cpu_qubits} qubits, {args.cpu_layers} layers, {threads} threads")

@@ -248,6 +248,44 @@ typedef struct mh_calibration {
 
 int mh_calibrate(mh_calibration* out);
 
+/* ---- Batched execution: many independent circuits per device launch.
+ * Replaces the reference's sequential batch (run_batch_with_options,
+ * src/api/batch.cpp:10-27, one full run per circuit with no device
+ * residency) for expectation-value workloads (the Estimator's path,
+ * python/menos/qiskit/estimator.py:91-133). Every circuit is planned and
+ * compiled on host threads; the states of a round of circuits share one
+ * device arena; pass p of every circuit in the round runs in one launch;
+ * every circuit's terms are reduced in two launches. Results are the
+ * single-circuit engine's bit for bit: complex128 statevectors as the
+ * reference's, expectation terms as mh_expectation's. -------------------- */
+typedef struct mh_batch_request {
+    uint32_t num_circuits;
+    const mh_circuit* circuits;    /* each with at least 10 (complex128) / 11 (complex64) qubits */
+    int32_t precision;
+    int32_t fusion;                /* mh_fusion */
+    uint32_t fusion_width;
+    const double* global_phases;   /* per circuit, or null (all 0) */
+    const uint32_t* term_offsets;  /* num_circuits + 1: circuit c owns terms [term_offsets[c], term_offsets[c+1]) */
+    const uint64_t* x_masks;
+    const uint64_t* z_masks;
+    uint64_t round_bytes;          /* device bytes of one round's states; 0: half the available memory */
+    uint32_t host_threads;         /* planning threads; 0: the hardware's, at most 16 */
+    int32_t want_statevectors;     /* copy every final state into result->statevectors */
+} mh_batch_request;
+
+typedef struct mh_batch_result {
+    double* expectation_terms;     /* 2 doubles per term, in term order */
+    void* statevectors;            /* want_statevectors: the states back to back, in circuit order */
+    double* completion_seconds;    /* per circuit, or null: from the call's start to its results on the host */
+    uint32_t rounds;
+    uint32_t launches;
+    double plan_seconds;           /* host: fusion plans and block matrices */
+    double device_seconds;         /* device: init, passes and reductions of every round */
+    double total_seconds;
+} mh_batch_result;
+
+int mh_batch_execute(const mh_batch_request* request, mh_batch_result* result);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,8 @@
 // reference's sequential index-order sum by rounding only (the reference's
 // own CUDA contract, dense_kernels.h:237-246).
 #include <algorithm>
+#include <cstdint>
+#include <cstring>
 #include <map>
 
 #include "mh_internal.h"
@@ -45,19 +47,21 @@ __device__ __forceinline__ void load_amp(const V* s, uint64_t i, double& re, dou
     im = static_cast<double>(v.y);
 }
 
+// Workgroup `block` of `blocks` for one group of terms sharing an X mask:
+// writes partials[(t * blocks + block) * 2 + {0, 1}] for t < terms.
 template <typename V, uint32_t T>
-__global__ void __launch_bounds__(kThreads) expectation_kernel(ExpArgs a) {
+__device__ __forceinline__ void expectation_group(const V* s, double* partials, uint64_t pairs, uint64_t x, uint32_t h,
+                                                  uint32_t terms, const uint64_t* z, uint32_t block, uint32_t blocks) {
     __shared__ double red[2][kThreads];
-    const V* s = static_cast<const V*>(a.state);
     // A fixed grid-stride assignment (coalesced; the same for every run).
-    const uint64_t total_threads = uint64_t(a.blocks) * kThreads;
-    const uint64_t tid = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+    const uint64_t total_threads = uint64_t(blocks) * kThreads;
+    const uint64_t tid = uint64_t(block) * kThreads + threadIdx.x;
     double accr[T], acci[T];
 #pragma unroll
     for (uint32_t t = 0; t < T; ++t) accr[t] = acci[t] = 0.0;
 
-    if (a.x == 0) {
-        for (uint64_t m = tid; m < a.pairs; m += total_threads) {
+    if (x == 0) {
+        for (uint64_t m = tid; m < pairs; m += total_threads) {
             double ar, ai;
             load_amp(s, m, ar, ai);
             // conj(a) * a: real ar*ar + ai*ai, imaginary ar*ai - ai*ar (exactly 0).
@@ -65,17 +69,17 @@ __global__ void __launch_bounds__(kThreads) expectation_kernel(ExpArgs a) {
             const double pi = ar * ai - ai * ar;
 #pragma unroll
             for (uint32_t t = 0; t < T; ++t) {
-                if (t >= a.terms) break;
-                const bool neg = (__popcll(a.z[t] & m) & 1) != 0;
+                if (t >= terms) break;
+                const bool neg = (__popcll(z[t] & m) & 1) != 0;
                 accr[t] = neg ? accr[t] - pr : accr[t] + pr;
                 acci[t] = neg ? acci[t] - pi : acci[t] + pi;
             }
         }
     } else {
-        const uint64_t lowm = (1ull << a.h) - 1ull;
-        for (uint64_t r = tid; r < a.pairs; r += total_threads) {
+        const uint64_t lowm = (1ull << h) - 1ull;
+        for (uint64_t r = tid; r < pairs; r += total_threads) {
             const uint64_t m = ((r & ~lowm) << 1) | (r & lowm);
-            const uint64_t mp = m ^ a.x;
+            const uint64_t mp = m ^ x;
             double ar, ai, br, bi;
             load_amp(s, m, ar, ai);
             load_amp(s, mp, br, bi);
@@ -84,9 +88,9 @@ __global__ void __launch_bounds__(kThreads) expectation_kernel(ExpArgs a) {
             const double pi = ar * bi - ai * br;
 #pragma unroll
             for (uint32_t t = 0; t < T; ++t) {
-                if (t >= a.terms) break;
-                const bool neg_mp = (__popcll(a.z[t] & mp) & 1) != 0;  // s(m') for p
-                const bool neg_m = (__popcll(a.z[t] & m) & 1) != 0;    // s(m) for conj(p)
+                if (t >= terms) break;
+                const bool neg_mp = (__popcll(z[t] & mp) & 1) != 0;  // s(m') for p
+                const bool neg_m = (__popcll(z[t] & m) & 1) != 0;    // s(m) for conj(p)
                 accr[t] = neg_mp ? accr[t] - pr : accr[t] + pr;
                 acci[t] = neg_mp ? acci[t] - pi : acci[t] + pi;
                 accr[t] = neg_m ? accr[t] - pr : accr[t] + pr;
@@ -95,7 +99,7 @@ __global__ void __launch_bounds__(kThreads) expectation_kernel(ExpArgs a) {
         }
     }
     // Fixed-order tree over the workgroup, one term at a time.
-    for (uint32_t t = 0; t < a.terms && t < T; ++t) {
+    for (uint32_t t = 0; t < terms && t < T; ++t) {
         double vr = 0.0, vi = 0.0;
 #pragma unroll
         for (uint32_t u = 0; u < T; ++u)
@@ -111,23 +115,67 @@ __global__ void __launch_bounds__(kThreads) expectation_kernel(ExpArgs a) {
             __syncthreads();
         }
         if (threadIdx.x == 0) {
-            a.partials[(uint64_t(t) * a.blocks + blockIdx.x) * 2] = red[0][0];
-            a.partials[(uint64_t(t) * a.blocks + blockIdx.x) * 2 + 1] = red[1][0];
+            partials[(uint64_t(t) * blocks + block) * 2] = red[0][0];
+            partials[(uint64_t(t) * blocks + block) * 2 + 1] = red[1][0];
         }
         __syncthreads();
     }
 }
 
-// One workgroup per term: the block partials in a fixed tree order, then the
-// phase i^popcount(x & z) as a complex product (reference dense_kernels.cpp:568).
-__global__ void __launch_bounds__(kThreads) expectation_finish_kernel(const double* partials, uint32_t blocks,
-                                                                      const int32_t* phase_k, double* out) {
+template <typename V, uint32_t T>
+__global__ void __launch_bounds__(kThreads) expectation_kernel(ExpArgs a) {
+    expectation_group<V, T>(static_cast<const V*>(a.state), a.partials, a.pairs, a.x, a.h, a.terms, a.z, blockIdx.x,
+                            a.blocks);
+}
+
+// One group of terms of one state in a batched launch; offsets count from the
+// launch's base pointers (amplitudes, doubles).
+struct ExpJob {
+    uint64_t state_off;
+    uint64_t pairs;
+    uint64_t x;
+    uint32_t h, terms, blocks, pad;
+    uint64_t partial_off;
+    uint64_t z[kTermsPerLaunch];
+};
+
+// Workgroup b belongs to job j with block_begin[j] <= b < block_begin[j + 1].
+template <typename V>
+__global__ void __launch_bounds__(kThreads)
+    expectation_batch_kernel(const ExpJob* __restrict__ jobs, const uint32_t* __restrict__ block_begin, uint32_t count,
+                             const V* base, double* partials) {
+    const uint32_t b = blockIdx.x;
+    uint32_t lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (block_begin[mid] <= b) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    const ExpJob& j = jobs[lo];
+    expectation_group<V, kTermsPerLaunch>(base + j.state_off, partials + j.partial_off, j.pairs, j.x, j.h, j.terms,
+                                          j.z, b - block_begin[lo], j.blocks);
+}
+
+// One term slot of a batched launch.
+struct ExpSlot {
+    uint64_t partial_off;    // the term's first partial (doubles)
+    uint32_t blocks;
+    int32_t phase_k;         // popcount(x & z)
+    uint32_t out;            // output term index
+    uint32_t pad;
+};
+
+// The block partials of one term in a fixed tree order, then the phase
+// i^popcount(x & z) as a complex product (reference dense_kernels.cpp:568).
+__device__ __forceinline__ void finish_term(const double* partials, uint32_t blocks, int32_t phase_k, double* out) {
     __shared__ double red[2][kThreads];
-    const uint32_t t = blockIdx.x;
     double vr = 0.0, vi = 0.0;
     for (uint32_t b = threadIdx.x; b < blocks; b += kThreads) {
-        vr = vr + partials[(uint64_t(t) * blocks + b) * 2];
-        vi = vi + partials[(uint64_t(t) * blocks + b) * 2 + 1];
+        vr = vr + partials[uint64_t(b) * 2];
+        vi = vi + partials[uint64_t(b) * 2 + 1];
     }
     red[0][threadIdx.x] = vr;
     red[1][threadIdx.x] = vi;
@@ -140,13 +188,30 @@ __global__ void __launch_bounds__(kThreads) expectation_finish_kernel(const doub
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const int k = phase_k[t] & 3;
+        const int k = phase_k & 3;
         const double fr = k == 0 ? 1.0 : (k == 2 ? -1.0 : 0.0);
         const double fi = k == 1 ? 1.0 : (k == 3 ? -1.0 : 0.0);
         const double sr = red[0][0], si = red[1][0];
-        out[2 * t] = fr * sr - fi * si;
-        out[2 * t + 1] = fr * si + fi * sr;
+        out[0] = fr * sr - fi * si;
+        out[1] = fr * si + fi * sr;
     }
+}
+
+// One workgroup per term.
+__global__ void __launch_bounds__(kThreads) expectation_finish_kernel(const double* partials, uint32_t blocks,
+                                                                      const int32_t* phase_k, double* out) {
+    const uint32_t t = blockIdx.x;
+    finish_term(partials + uint64_t(t) * blocks * 2, blocks, phase_k[t], out + 2 * t);
+}
+
+__global__ void __launch_bounds__(kThreads) expectation_finish_batch_kernel(const ExpSlot* __restrict__ slots,
+                                                                            const double* partials, double* out) {
+    const ExpSlot& s = slots[blockIdx.x];
+    finish_term(partials + s.partial_off, s.blocks, s.phase_k, out + 2 * uint64_t(s.out));
+}
+
+uint32_t blocks_for(uint64_t pairs) {
+    return static_cast<uint32_t>(std::min<uint64_t>(kMaxBlocks, std::max<uint64_t>(1, pairs / (kThreads * 16))));
 }
 
 template <typename V>
@@ -160,7 +225,7 @@ void run_group(const void* state, uint32_t n, uint64_t x, const std::vector<uint
     a.h = x ? 63u - static_cast<uint32_t>(__builtin_clzll(x)) : 0u;
     a.pairs = x ? (uint64_t{1} << (n - 1)) : (uint64_t{1} << n);
     a.terms = terms;
-    a.blocks = static_cast<uint32_t>(std::min<uint64_t>(kMaxBlocks, std::max<uint64_t>(1, a.pairs / (kThreads * 16))));
+    a.blocks = blocks_for(a.pairs);
     for (uint32_t t = 0; t < terms; ++t) a.z[t] = z[idx[t]];
     DeviceBuffer partials(uint64_t(a.blocks) * terms * 2 * sizeof(double));
     DeviceBuffer res(uint64_t(terms) * 2 * sizeof(double) + terms * sizeof(int32_t));
@@ -211,6 +276,88 @@ void expectation_terms(const void* state, uint32_t n, int32_t precision, const u
                 run_group<float2>(state, n, xm, part, z, o);
         }
     }
+}
+
+template <typename V>
+void expectation_batch_typed(const std::vector<ExpectationItem>& items, double* out) {
+    Context& ctx = context();
+    uintptr_t base_addr = UINTPTR_MAX;
+    for (const ExpectationItem& it : items) base_addr = std::min(base_addr, reinterpret_cast<uintptr_t>(it.state));
+    std::vector<ExpJob> jobs;
+    std::vector<uint32_t> block_begin;
+    std::vector<ExpSlot> slots;
+    uint64_t partial_doubles = 0;
+    uint32_t total_blocks = 0;
+    for (const ExpectationItem& it : items) {
+        if (it.n == 0 || it.n > 40) fail(MH_ERR_INVALID, "expectation: num_qubits must be 1..40");
+        const uint64_t limit = (uint64_t{1} << it.n) - 1;
+        std::map<uint64_t, std::vector<uint32_t>> groups;
+        for (uint32_t t = 0; t < it.terms; ++t) {
+            if ((it.x[t] & ~limit) || (it.z[t] & ~limit)) fail(MH_ERR_INVALID, "a Pauli mask names a qubit beyond the state");
+            groups[it.x[t]].push_back(t);
+        }
+        const uintptr_t addr = reinterpret_cast<uintptr_t>(it.state);
+        for (auto& [xm, members] : groups) {
+            for (std::size_t at = 0; at < members.size(); at += kTermsPerLaunch) {
+                const std::size_t cnt = std::min<std::size_t>(kTermsPerLaunch, members.size() - at);
+                ExpJob j;
+                std::memset(&j, 0, sizeof(j));
+                j.state_off = (addr - base_addr) / sizeof(V);
+                j.x = xm;
+                j.h = xm ? 63u - static_cast<uint32_t>(__builtin_clzll(xm)) : 0u;
+                j.pairs = xm ? (uint64_t{1} << (it.n - 1)) : (uint64_t{1} << it.n);
+                j.terms = static_cast<uint32_t>(cnt);
+                j.blocks = blocks_for(j.pairs);
+                j.partial_off = partial_doubles;
+                for (std::size_t k = 0; k < cnt; ++k) {
+                    const uint32_t t = members[at + k];
+                    j.z[k] = it.z[t];
+                    ExpSlot sl;
+                    std::memset(&sl, 0, sizeof(sl));
+                    sl.partial_off = partial_doubles + uint64_t(k) * j.blocks * 2;
+                    sl.blocks = j.blocks;
+                    sl.phase_k = __builtin_popcountll(xm & it.z[t]);
+                    sl.out = it.out_index + t;
+                    slots.push_back(sl);
+                }
+                partial_doubles += uint64_t(cnt) * j.blocks * 2;
+                block_begin.push_back(total_blocks);
+                total_blocks += j.blocks;
+                jobs.push_back(j);
+            }
+        }
+    }
+    if (jobs.empty()) return;
+    uint32_t out_terms = 0;
+    for (const ExpSlot& sl : slots) out_terms = std::max(out_terms, sl.out + 1);
+    DeviceBuffer djobs(jobs.size() * sizeof(ExpJob)), dbegin(block_begin.size() * sizeof(uint32_t));
+    DeviceBuffer dslots(slots.size() * sizeof(ExpSlot)), dpartials(partial_doubles * sizeof(double));
+    DeviceBuffer dout(uint64_t(out_terms) * 2 * sizeof(double));
+    upload(djobs.get(), jobs.data(), jobs.size() * sizeof(ExpJob));
+    upload(dbegin.get(), block_begin.data(), block_begin.size() * sizeof(uint32_t));
+    upload(dslots.get(), slots.data(), slots.size() * sizeof(ExpSlot));
+    hipLaunchKernelGGL(expectation_batch_kernel<V>, dim3(total_blocks), dim3(kThreads), 0, ctx.stream,
+                       static_cast<const ExpJob*>(djobs.as<ExpJob>()), static_cast<const uint32_t*>(dbegin.as<uint32_t>()),
+                       static_cast<uint32_t>(jobs.size()), reinterpret_cast<const V*>(base_addr), dpartials.as<double>());
+    MH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(expectation_finish_batch_kernel, dim3(static_cast<uint32_t>(slots.size())), dim3(kThreads), 0,
+                       ctx.stream, static_cast<const ExpSlot*>(dslots.as<ExpSlot>()),
+                       static_cast<const double*>(dpartials.as<double>()), dout.as<double>());
+    MH_HIP(hipGetLastError());
+    std::vector<double> host(uint64_t(out_terms) * 2);
+    download(host.data(), dout.get(), host.size() * sizeof(double));
+    for (const ExpSlot& sl : slots) {
+        out[2 * uint64_t(sl.out)] = host[2 * uint64_t(sl.out)];
+        out[2 * uint64_t(sl.out) + 1] = host[2 * uint64_t(sl.out) + 1];
+    }
+}
+
+void expectation_batch(int32_t precision, const std::vector<ExpectationItem>& items, double* out) {
+    amp_bytes(precision);
+    if (precision == MH_C128)
+        expectation_batch_typed<double2>(items, out);
+    else
+        expectation_batch_typed<float2>(items, out);
 }
 
 }  // namespace mh

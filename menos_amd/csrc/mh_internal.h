@@ -128,6 +128,17 @@ bool matrix_is_diagonal(const cplx* m, std::size_t dim);
 void expectation_terms(const void* state, uint32_t n, int32_t precision, const uint64_t* x, const uint64_t* z,
                        uint32_t terms, cplx* out);
 
+// The terms of many states in two launches; results land at out[2 * (out_index + t)].
+struct ExpectationItem {
+    const void* state = nullptr;
+    uint32_t n = 0;
+    uint32_t terms = 0;
+    const uint64_t* x = nullptr;
+    const uint64_t* z = nullptr;
+    uint32_t out_index = 0;
+};
+void expectation_batch(int32_t precision, const std::vector<ExpectationItem>& items, double* out);
+
 // ---- sampler (sampler.hip)
 void sample_sorted(const void* state, uint32_t n, int32_t precision, const double* sorted_u, uint64_t shots,
                    uint64_t* basis_out);

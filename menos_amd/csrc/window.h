@@ -31,6 +31,24 @@ std::vector<WindowPass> plan_window_passes(uint32_t n, int32_t precision, const 
 void run_window_passes(void* state, uint32_t n, int32_t precision, const std::vector<Block>& blocks, uint32_t w,
                        uint32_t max_cost, bool time_passes, PassStats* stats);
 
+// One state of a batch: its amplitudes and the blocks to apply to them.
+struct BatchItem {
+    void* state = nullptr;
+    uint32_t n = 0;
+    const std::vector<Block>* blocks = nullptr;
+};
+
+struct BatchStats {
+    uint32_t launches = 0;
+    uint32_t max_passes = 0;
+};
+
+// Applies every item's blocks to its state (default window and pass cap):
+// pass p of all items in one launch. Plans and compiles on `threads` host
+// threads; synchronises. Every item needs at least window_width() qubits.
+void run_window_passes_batch(int32_t precision, const std::vector<BatchItem>& items, uint32_t threads,
+                             BatchStats* stats);
+
 // The generic one-thread-per-group kernel for one block (gates.hip); synchronises.
 void launch_generic(void* state, uint32_t n, int32_t precision, const cplx* matrix, const uint32_t* operands,
                     uint32_t k);

@@ -34,7 +34,12 @@
 // adding +0 to every component at the end of the last pass gives its bits
 // exactly. Complex64 passes use fused multiply-adds and carry a stated error.
 #include <algorithm>
+#include <atomic>
+#include <cstdint>
 #include <cstring>
+#include <exception>
+#include <mutex>
+#include <thread>
 
 #include "mh_internal.h"
 #include "window.h"
@@ -110,25 +115,38 @@ struct SignTerm {
     uint8_t pad[3];
 };
 
+// Where a pass's window qubits live at the load and the store, and which
+// outside qubits the wave index selects.
+struct WLayout {
+    uint32_t num_bundles;
+    uint32_t normalize;       // add +0 to every component before the store
+    uint32_t num_outer;       // qubits outside the window
+    uint32_t num_waves;       // windows of the state: 2^(n - W)
+    uint8_t outer_q[32];      // wave index bit b -> global qubit outer_q[b]
+    uint8_t load_lane_q[8], load_reg_q[8];    // global qubit of each lane / register bit at the load
+    uint8_t store_lane_q[8], store_reg_q[8];  // ... and at the store
+};
+
 struct WPassArgs {
     void* state;
     const uint64_t* bundles;
     const WOp* ops;
     const SignTerm* terms;
     const void* mats;
-    uint32_t num_bundles;
-    uint32_t normalize;       // add +0 to every component before the store
-    uint32_t num_outer;       // qubits outside the window
-    uint32_t pad;
-    uint8_t outer_q[32];      // wave index bit b -> global qubit outer_q[b]
-    uint8_t load_lane_q[8], load_reg_q[8];    // global qubit of each lane / register bit at the load
-    uint8_t store_lane_q[8], store_reg_q[8];  // ... and at the store
+    WLayout lay;
+};
+
+// One state's pass in a batched launch: offsets from the launch's base pointers.
+struct WBatchArgs {
+    uint64_t state_off;       // amplitudes
+    uint32_t bundle_off, op_off, term_off, mat_off;
+    WLayout lay;
 };
 
 // The qubits outside the window, taken from the wave index (a bit deposit).
-__device__ __forceinline__ uint64_t outer_base(uint32_t wave, const WPassArgs& a) {
+__device__ __forceinline__ uint64_t outer_base(uint32_t wave, const WLayout& lay) {
     uint64_t base = 0;
-    for (uint32_t b = 0; b < a.num_outer; ++b) base |= uint64_t((wave >> b) & 1u) << a.outer_q[b];
+    for (uint32_t b = 0; b < lay.num_outer; ++b) base |= uint64_t((wave >> b) & 1u) << lay.outer_q[b];
     return base;
 }
 
@@ -439,17 +457,16 @@ __device__ __forceinline__ void reg_swap_stage(uint64_t c, V* v) {
         if (c & (1ull << Stages::reg_swap(J1, J2, RB))) reg_swap<V, E, J1, J2>(v);
 }
 
-// Four waves per SIMD: each wave holds its window in at most 128 VGPRs.
+// One wave's window of one pass.
 template <typename R>
-__global__ void __launch_bounds__(kWavesPerGroup * 64, 4) window_kernel(WPassArgs a) {
+__device__ __forceinline__ void window_body(typename Shape<R>::V* state, const uint64_t* bundles, const WOp* ops,
+                                            const SignTerm* terms, const typename Shape<R>::V* mats,
+                                            const WLayout& a, uint32_t wave, uint32_t lane) {
     using V = typename Shape<R>::V;
     constexpr uint32_t RB = Shape<R>::kRegBits;
     constexpr uint32_t E = 1u << RB;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     const uint64_t obase = outer_base(wave, a);
-    V* const st = static_cast<V*>(a.state) + obase;
-    const V* const mats = static_cast<const V*>(a.mats);
+    V* const st = state + obase;
 
     V v[E];
     {
@@ -467,10 +484,10 @@ __global__ void __launch_bounds__(kWavesPerGroup * 64, 4) window_kernel(WPassArg
         }
     }
 
-    const WOp* op = a.ops;
+    const WOp* op = ops;
     for (uint32_t b = 0; b < a.num_bundles; ++b) {
-        const uint64_t c = a.bundles[b];
-        if (c & (1ull << Stages::kSignEarly)) apply_signs<V, E>(v, *op++, a.terms, obase, lane);
+        const uint64_t c = bundles[b];
+        if (c & (1ull << Stages::kSignEarly)) apply_signs<V, E>(v, *op++, terms, obase, lane);
         if (c & (1ull << Stages::kDiagEarly)) apply_diag<V, E>(v, *op++, mats, obase, lane);
         group_stages<V, E, 0>(c, v, op, mats, lane);
         group_stages<V, E, 1>(c, v, op, mats, lane);
@@ -478,7 +495,7 @@ __global__ void __launch_bounds__(kWavesPerGroup * 64, 4) window_kernel(WPassArg
         group_stages<V, E, 3>(c, v, op, mats, lane);
         group_stages<V, E, 4>(c, v, op, mats, lane);
         if (c & (1ull << Stages::dense2(RB))) dense2<E>(v, mats + (op++)->mat);
-        if (c & (1ull << Stages::sign_late(RB))) apply_signs<V, E>(v, *op++, a.terms, obase, lane);
+        if (c & (1ull << Stages::sign_late(RB))) apply_signs<V, E>(v, *op++, terms, obase, lane);
         if (c & (1ull << Stages::diag_late(RB))) apply_diag<V, E>(v, *op++, mats, obase, lane);
         if (c >> Stages::reg_swap(0, 1, RB)) {
             reg_swap_stage<V, E, RB, 0, 1>(c, v);
@@ -519,6 +536,42 @@ __global__ void __launch_bounds__(kWavesPerGroup * 64, 4) window_kernel(WPassArg
             store_amp(dst + h, v[r]);
         }
     }
+}
+
+// Four waves per SIMD: each wave holds its window in at most 128 VGPRs.
+template <typename R>
+__global__ void __launch_bounds__(kWavesPerGroup * 64, 4) window_kernel(WPassArgs a) {
+    using V = typename Shape<R>::V;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    window_body<R>(static_cast<V*>(a.state), a.bundles, a.ops, a.terms, static_cast<const V*>(a.mats), a.lay, wave,
+                   threadIdx.x & 63u);
+}
+
+// One pass of many states in one launch: workgroup b belongs to the state c
+// with group_begin[c] <= b < group_begin[c + 1] (each state's windows in
+// groups of four waves); a wave past its state's last window returns.
+template <typename R>
+__global__ void __launch_bounds__(kWavesPerGroup * 64, 4)
+    window_batch_kernel(const WBatchArgs* __restrict__ args, const uint32_t* __restrict__ group_begin, uint32_t count,
+                        typename Shape<R>::V* state, const uint64_t* __restrict__ bundles,
+                        const WOp* __restrict__ ops, const SignTerm* __restrict__ terms,
+                        const typename Shape<R>::V* __restrict__ mats) {
+    const uint32_t b = blockIdx.x;
+    uint32_t lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (group_begin[mid] <= b) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    const WBatchArgs& a = args[lo];
+    const uint32_t wave =
+        __builtin_amdgcn_readfirstlane((b - group_begin[lo]) * kWavesPerGroup + (threadIdx.x >> 6));
+    if (wave >= a.lay.num_waves) return;
+    window_body<R>(state + a.state_off, bundles + a.bundle_off, ops + a.op_off, terms + a.term_off,
+                   mats + a.mat_off, a.lay, wave, threadIdx.x & 63u);
 }
 
 // ---- host: passes
@@ -594,14 +647,14 @@ public:
             for (uint32_t q = 0; q < n; ++q)
                 if (!((wmask >> q) & 1ull)) outside[count_out++] = static_cast<uint8_t>(q);
             const uint32_t head = std::min<uint32_t>(3, count_out);
-            for (uint32_t k = 0; k < head; ++k) a.outer_q[a.num_outer++] = outside[k];
-            for (uint32_t k = count_out; k-- > head;) a.outer_q[a.num_outer++] = outside[k];
+            for (uint32_t k = 0; k < head; ++k) a.lay.outer_q[a.lay.num_outer++] = outside[k];
+            for (uint32_t k = count_out; k-- > head;) a.lay.outer_q[a.lay.num_outer++] = outside[k];
         }
         blocks_.clear();
         for (uint32_t b : pass.blocks) blocks_.push_back(&blocks[b]);
         initial_layout();
-        for (uint32_t i = 0; i < kLaneBits; ++i) a.load_lane_q[i] = wq_[lane_[i]];
-        for (uint32_t j = 0; j < RB; ++j) a.load_reg_q[j] = wq_[reg_[j]];
+        for (uint32_t i = 0; i < kLaneBits; ++i) a.lay.load_lane_q[i] = wq_[lane_[i]];
+        for (uint32_t j = 0; j < RB; ++j) a.lay.load_reg_q[j] = wq_[reg_[j]];
 
         for (std::size_t i = 0; i < blocks_.size(); ++i) {
             const Block& b = *blocks_[i];
@@ -648,9 +701,9 @@ public:
             lane_swap(l, static_cast<uint32_t>(j));
         }
         flush();
-        for (uint32_t i = 0; i < kLaneBits; ++i) a.store_lane_q[i] = wq_[lane_[i]];
-        for (uint32_t j = 0; j < RB; ++j) a.store_reg_q[j] = wq_[reg_[j]];
-        a.normalize = normalize ? 1u : 0u;
+        for (uint32_t i = 0; i < kLaneBits; ++i) a.lay.store_lane_q[i] = wq_[lane_[i]];
+        for (uint32_t j = 0; j < RB; ++j) a.lay.store_reg_q[j] = wq_[reg_[j]];
+        a.lay.normalize = normalize ? 1u : 0u;
         return a;
     }
 
@@ -888,7 +941,7 @@ void launch_window_passes(void* state, uint32_t n, const std::vector<Block>& blo
         if (passes[p].generic) continue;
         const bool last = p + 1 == passes.size();
         args[p] = compiler.compile(n, blocks, passes[p], last && sizeof(R) == 8);
-        args[p].num_bundles = static_cast<uint32_t>(bundles.size()) - bundle_begin[p];
+        args[p].lay.num_bundles = static_cast<uint32_t>(bundles.size()) - bundle_begin[p];
     }
     DeviceBuffer dbundles(std::max<std::size_t>(bundles.size(), 1) * sizeof(uint64_t));
     DeviceBuffer dops(std::max<std::size_t>(ops.size(), 1) * sizeof(WOp));
@@ -932,6 +985,166 @@ void launch_window_passes(void* state, uint32_t n, const std::vector<Block>& blo
         for (std::size_t p = 0; p < passes.size(); ++p)
             MH_HIP(hipEventElapsedTime(&stats->pass_ms[p], events[p], events[p + 1]));
         for (auto& e : events) (void)hipEventDestroy(e);
+    }
+}
+
+// Many states, each with its own block list: pass p of every state runs in
+// one launch of window_batch_kernel (blocks the window kernel cannot apply run
+// on the generic kernel, state by state, after that launch).
+template <typename R>
+void launch_window_passes_batch(const std::vector<BatchItem>& items, uint32_t threads, BatchStats* stats) {
+    using V = typename Shape<R>::V;
+    constexpr uint32_t W = width_of<R>();
+    Context& ctx = context();
+    struct PassRec {
+        WPassArgs args;
+        uint32_t bundle_begin = 0, op_begin = 0;
+        bool generic = false;
+        uint32_t generic_block = 0;
+    };
+    struct Compiled {
+        std::vector<uint64_t> bundles;
+        std::vector<WOp> ops;
+        std::vector<SignTerm> terms;
+        std::vector<V> mats;
+        std::vector<PassRec> passes;
+    };
+    const std::size_t count = items.size();
+    std::vector<Compiled> comp(count);
+    std::atomic<std::size_t> next{0};
+    std::exception_ptr error;
+    std::mutex error_mutex;
+    auto worker = [&] {
+        try {
+            for (std::size_t i; (i = next.fetch_add(1)) < count;) {
+                const BatchItem& it = items[i];
+                Compiled& c = comp[i];
+                const std::vector<WindowPass> passes =
+                    plan_window_passes(it.n, sizeof(R) == 8 ? MH_C128 : MH_C64, *it.blocks, W, kDefaultPassCost);
+                PassCompiler<R> compiler(c.bundles, c.ops, c.terms, c.mats);
+                for (std::size_t p = 0; p < passes.size(); ++p) {
+                    PassRec rec;
+                    rec.bundle_begin = static_cast<uint32_t>(c.bundles.size());
+                    rec.op_begin = static_cast<uint32_t>(c.ops.size());
+                    if (passes[p].generic) {
+                        rec.generic = true;
+                        rec.generic_block = passes[p].blocks[0];
+                    } else {
+                        rec.args = compiler.compile(it.n, *it.blocks, passes[p], p + 1 == passes.size() && sizeof(R) == 8);
+                        rec.args.lay.num_bundles = static_cast<uint32_t>(c.bundles.size()) - rec.bundle_begin;
+                        rec.args.lay.num_waves = 1u << (it.n - W);
+                    }
+                    c.passes.push_back(rec);
+                }
+            }
+        } catch (...) {
+            std::lock_guard<std::mutex> lock(error_mutex);
+            if (!error) error = std::current_exception();
+        }
+    };
+    {
+        std::vector<std::thread> pool;
+        const uint32_t t = std::max<uint32_t>(1, std::min<uint32_t>(threads, static_cast<uint32_t>(count)));
+        for (uint32_t k = 1; k < t; ++k) pool.emplace_back(worker);
+        worker();
+        for (auto& th : pool) th.join();
+    }
+    if (error) std::rethrow_exception(error);
+
+    // One device copy of every state's tables.
+    std::vector<std::size_t> bundle_base(count), op_base(count), term_base(count), mat_base(count);
+    std::size_t nb = 0, no = 0, nt = 0, nm = 0, max_passes = 0;
+    for (std::size_t i = 0; i < count; ++i) {
+        bundle_base[i] = nb;
+        op_base[i] = no;
+        term_base[i] = nt;
+        mat_base[i] = nm;
+        nb += comp[i].bundles.size();
+        no += comp[i].ops.size();
+        nt += comp[i].terms.size();
+        nm += comp[i].mats.size();
+        max_passes = std::max(max_passes, comp[i].passes.size());
+    }
+    std::vector<uint64_t> bundles(std::max<std::size_t>(nb, 1));
+    std::vector<WOp> ops(std::max<std::size_t>(no, 1));
+    std::vector<SignTerm> terms(std::max<std::size_t>(nt, 1));
+    std::vector<V> mats(std::max<std::size_t>(nm, 1));
+    for (std::size_t i = 0; i < count; ++i) {
+        std::copy(comp[i].bundles.begin(), comp[i].bundles.end(), bundles.begin() + bundle_base[i]);
+        std::copy(comp[i].ops.begin(), comp[i].ops.end(), ops.begin() + op_base[i]);
+        std::copy(comp[i].terms.begin(), comp[i].terms.end(), terms.begin() + term_base[i]);
+        std::copy(comp[i].mats.begin(), comp[i].mats.end(), mats.begin() + mat_base[i]);
+    }
+    DeviceBuffer dbundles(bundles.size() * sizeof(uint64_t)), dops(ops.size() * sizeof(WOp));
+    DeviceBuffer dterms(terms.size() * sizeof(SignTerm)), dmats(mats.size() * sizeof(V));
+    upload(dbundles.get(), bundles.data(), bundles.size() * sizeof(uint64_t));
+    upload(dops.get(), ops.data(), ops.size() * sizeof(WOp));
+    upload(dterms.get(), terms.data(), terms.size() * sizeof(SignTerm));
+    upload(dmats.get(), mats.data(), mats.size() * sizeof(V));
+
+    // Launch lists: pass p of every state that has one. State offsets count
+    // amplitudes from the lowest state address.
+    uintptr_t base_addr = UINTPTR_MAX;
+    for (const BatchItem& it : items) base_addr = std::min(base_addr, reinterpret_cast<uintptr_t>(it.state));
+    V* const state_base = reinterpret_cast<V*>(base_addr);
+    std::vector<WBatchArgs> all_args;
+    std::vector<uint32_t> all_groups;
+    std::vector<std::size_t> args_at(max_passes), groups_at(max_passes), list_len(max_passes);
+    std::vector<uint32_t> total_groups(max_passes);
+    for (std::size_t p = 0; p < max_passes; ++p) {
+        args_at[p] = all_args.size();
+        groups_at[p] = all_groups.size();
+        uint32_t g = 0;
+        for (std::size_t i = 0; i < count; ++i) {
+            if (p >= comp[i].passes.size() || comp[i].passes[p].generic) continue;
+            const PassRec& rec = comp[i].passes[p];
+            const uintptr_t addr = reinterpret_cast<uintptr_t>(items[i].state);
+            if ((addr - base_addr) % sizeof(V) != 0) fail(MH_ERR_INVALID, "a batched state is not aligned to its amplitudes");
+            WBatchArgs a;
+            std::memset(&a, 0, sizeof(a));
+            a.state_off = (addr - base_addr) / sizeof(V);
+            a.bundle_off = static_cast<uint32_t>(bundle_base[i] + rec.bundle_begin);
+            a.op_off = static_cast<uint32_t>(op_base[i] + rec.op_begin);
+            a.term_off = static_cast<uint32_t>(term_base[i]);
+            a.mat_off = static_cast<uint32_t>(mat_base[i]);
+            a.lay = rec.args.lay;
+            all_args.push_back(a);
+            all_groups.push_back(g);
+            g += (a.lay.num_waves + kWavesPerGroup - 1) / kWavesPerGroup;
+        }
+        list_len[p] = all_args.size() - args_at[p];
+        total_groups[p] = g;
+    }
+    DeviceBuffer dargs(std::max<std::size_t>(all_args.size(), 1) * sizeof(WBatchArgs));
+    DeviceBuffer dgroups(std::max<std::size_t>(all_groups.size(), 1) * sizeof(uint32_t));
+    upload(dargs.get(), all_args.data(), all_args.size() * sizeof(WBatchArgs));
+    upload(dgroups.get(), all_groups.data(), all_groups.size() * sizeof(uint32_t));
+
+    uint32_t launches = 0;
+    for (std::size_t p = 0; p < max_passes; ++p) {
+        if (list_len[p] > 0) {
+            hipLaunchKernelGGL(window_batch_kernel<R>, dim3(total_groups[p]), dim3(kWavesPerGroup * 64), 0, ctx.stream,
+                               static_cast<const WBatchArgs*>(dargs.as<WBatchArgs>() + args_at[p]),
+                               static_cast<const uint32_t*>(dgroups.as<uint32_t>() + groups_at[p]),
+                               static_cast<uint32_t>(list_len[p]), state_base,
+                               static_cast<const uint64_t*>(dbundles.as<uint64_t>()),
+                               static_cast<const WOp*>(dops.as<WOp>()), static_cast<const SignTerm*>(dterms.as<SignTerm>()),
+                               static_cast<const V*>(dmats.as<V>()));
+            MH_HIP(hipGetLastError());
+            ++launches;
+        }
+        for (std::size_t i = 0; i < count; ++i) {
+            if (p >= comp[i].passes.size() || !comp[i].passes[p].generic) continue;
+            const Block& blk = (*items[i].blocks)[comp[i].passes[p].generic_block];
+            launch_generic(items[i].state, items[i].n, sizeof(R) == 8 ? MH_C128 : MH_C64, blk.matrix.data(),
+                           blk.operands.data(), static_cast<uint32_t>(blk.operands.size()));
+            ++launches;
+        }
+    }
+    synchronize();
+    if (stats) {
+        stats->launches = launches;
+        stats->max_passes = static_cast<uint32_t>(max_passes);
     }
 }
 
@@ -993,6 +1206,18 @@ std::vector<WindowPass> plan_window_passes(uint32_t n, int32_t precision, const 
     }
     close();
     return passes;
+}
+
+void run_window_passes_batch(int32_t precision, const std::vector<BatchItem>& items, uint32_t threads,
+                             BatchStats* stats) {
+    if (items.empty()) return;
+    const uint32_t width = window_width(precision);
+    for (const BatchItem& it : items)
+        if (it.n < width) fail(MH_ERR_INVALID, "a batched state has fewer qubits than one window");
+    if (precision == MH_C128)
+        launch_window_passes_batch<double>(items, threads, stats);
+    else
+        launch_window_passes_batch<float>(items, threads, stats);
 }
 
 void run_window_passes(void* state, uint32_t n, int32_t precision, const std::vector<Block>& blocks, uint32_t w,

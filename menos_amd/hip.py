@@ -26,7 +26,7 @@ EXPORTS = (
     "mh_pool_reusable_bytes", "mh_pool_trim", "mh_state_init", "mh_apply_matrix", "mh_apply_matrix_generic",
     "mh_apply_blocks", "mh_expectation", "mh_sample", "mh_sample_unsorted", "mh_copy_to_host",
     "mh_copy_from_host", "mh_state_digest", "mh_engine_estimate", "mh_engine_execute", "mh_plan_fusion",
-    "mh_gate_matrix", "mh_calibrate",
+    "mh_gate_matrix", "mh_calibrate", "mh_batch_execute",
 )
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -314,6 +314,54 @@ class HipDenseEngine:
                       peak_device_bytes=res.peak_device_bytes, execute_seconds=res.execute_seconds,
                       gate_seconds=res.gate_seconds, sample_seconds=res.sample_seconds,
                       expectation_seconds=res.expectation_seconds)
+
+
+class CBatchRequest(ctypes.Structure):
+    _fields_ = [("num_circuits", ctypes.c_uint32), ("circuits", ctypes.POINTER(CCircuit)),
+                ("precision", ctypes.c_int32), ("fusion", ctypes.c_int32), ("fusion_width", ctypes.c_uint32),
+                ("global_phases", _f64p), ("term_offsets", _u32p), ("x_masks", _u64p), ("z_masks", _u64p),
+                ("round_bytes", ctypes.c_uint64), ("host_threads", ctypes.c_uint32),
+                ("want_statevectors", ctypes.c_int32)]
+
+
+class CBatchResult(ctypes.Structure):
+    _fields_ = [("expectation_terms", _f64p), ("statevectors", ctypes.c_void_p), ("completion_seconds", _f64p),
+                ("rounds", ctypes.c_uint32), ("launches", ctypes.c_uint32), ("plan_seconds", ctypes.c_double),
+                ("device_seconds", ctypes.c_double), ("total_seconds", ctypes.c_double)]
+
+
+def batch_execute(circuits, terms, *, precision=C128, fusion=FUSE_CONTAINMENT, fusion_width=0, round_bytes=0,
+                  host_threads=0, statevectors=False):
+    """Runs every circuit and reduces its Pauli terms (a list of (x, z) masks per
+    circuit) through mh_batch_execute. Returns a Result with `expectation` (one
+    complex array per circuit), `completion_seconds`, optionally `statevectors`
+    (one array per circuit) and the call's timings."""
+    keeps = [_CircuitKeep(c) for c in circuits]
+    arr = (CCircuit * max(len(keeps), 1))(*[k.c for k in keeps])
+    offs = np.zeros(len(circuits) + 1, np.uint32)
+    for i, t in enumerate(terms):
+        offs[i + 1] = offs[i] + len(t)
+    xs = np.array([x for t in terms for x, _ in t] or [0], np.uint64)
+    zs = np.array([z for t in terms for _, z in t] or [0], np.uint64)
+    out = np.zeros(2 * max(int(offs[-1]), 1), np.float64)
+    done = np.zeros(max(len(circuits), 1), np.float64)
+    sizes = [1 << c.num_qubits for c in circuits]
+    sv = np.zeros(sum(sizes) if statevectors else 1, amp_dtype(precision))
+    phases = np.array([c.global_phase for c in circuits] or [0.0], np.float64)
+    req = CBatchRequest(len(circuits), arr, precision, fusion, fusion_width, _ptr(phases, _f64p), _ptr(offs, _u32p),
+                        _ptr(xs, _u64p), _ptr(zs, _u64p), round_bytes, host_threads, 1 if statevectors else 0)
+    res = CBatchResult(_ptr(out, _f64p), sv.ctypes.data_as(ctypes.c_void_p), _ptr(done, _f64p))
+    check(lib().mh_batch_execute(ctypes.byref(req), ctypes.byref(res)))
+    e = out[:2 * int(offs[-1])].view(np.complex128)
+    r = Result(expectation=[e[offs[i]:offs[i + 1]].copy() for i in range(len(circuits))],
+               completion_seconds=done[:len(circuits)].copy(), rounds=res.rounds, launches=res.launches,
+               plan_seconds=res.plan_seconds, device_seconds=res.device_seconds, total_seconds=res.total_seconds)
+    if statevectors:
+        at, r.statevectors = 0, []
+        for s in sizes:
+            r.statevectors.append(sv[at:at + s].copy())
+            at += s
+    return r
 
 
 def calibrate() -> Calibration:

@@ -257,3 +257,52 @@ def digest_counts(counts: dict[int, int]) -> str:
         h = fnv1a(hex(key).encode(), h)
         h = fnv1a(int(counts[key]).to_bytes(8, "little"), h)
     return "%016x" % h
+
+
+class FlatAnsatz:
+    """``ansatz_circuit`` built directly in the flat layout (vectorised: a batch
+    of thousands of circuits is built in well under a second)."""
+
+    def __init__(self, num_qubits: int, depth: int, angles: np.ndarray):
+        n, d = int(num_qubits), int(depth)
+        self.num_qubits = n
+        self.num_clbits = 0
+        self.global_phase = 0.0
+        per_layer = 2 * n + (n - 1)
+        layer_ops = np.concatenate([np.full(n, RY), np.full(n, RZ), np.full(n - 1, CX)]).astype(np.int32)
+        self._opcodes = np.tile(layer_ops, d)
+        layer_q = [np.arange(n), np.arange(n)]
+        ladder = np.stack([np.arange(n - 1), np.arange(1, n)], axis=1).reshape(-1)
+        q_layer = np.concatenate([layer_q[0], layer_q[1], ladder]).astype(np.uint32)
+        self._qubits = np.tile(q_layer, d)
+        arity = np.concatenate([np.ones(2 * n, np.uint32), np.full(n - 1, 2, np.uint32)])
+        self._qo = np.concatenate([[0], np.cumsum(np.tile(arity, d))]).astype(np.uint32)
+        nparams = np.concatenate([np.ones(2 * n, np.uint32), np.zeros(n - 1, np.uint32)])
+        self._po = np.concatenate([[0], np.cumsum(np.tile(nparams, d))]).astype(np.uint32)
+        self._params = np.ascontiguousarray(angles, np.float64)
+        assert self._params.size == 2 * n * d and self._opcodes.size == per_layer * d
+        self.ops = range(self._opcodes.size)  # len() only
+
+    def flat(self):
+        return self._opcodes, self._qo, self._qubits, self._po, self._params
+
+    def measure_targets(self):
+        return []
+
+
+def batch_workload(count: int, seed: int, qubits=(10, 22), depth=(4, 12), terms: int = 8, max_weight: int = 4):
+    """The batched estimator workload: ``count`` ansatz circuits with n uniform
+    on ``qubits`` and depth uniform on ``depth`` (inclusive), angles uniform on
+    [-pi, pi) (numpy PCG64 from ``seed``), each with ``terms`` Pauli strings of
+    weight 1..``max_weight`` from ``random_paulis(n, terms, seed + i)``.
+    Returns (circuits, terms per circuit)."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(qubits[0], qubits[1] + 1, count)
+    ds = rng.integers(depth[0], depth[1] + 1, count)
+    circuits, paulis = [], []
+    for i in range(count):
+        n, d = int(ns[i]), int(ds[i])
+        angles = rng.uniform(-math.pi, math.pi, 2 * n * d)
+        circuits.append(FlatAnsatz(n, d, angles))
+        paulis.append(random_paulis(n, terms, seed + i, max_weight=max_weight, min_weight=1))
+    return circuits, paulis

@@ -14,14 +14,14 @@ def random_unitary(k: int, rng: np.random.Generator) -> np.ndarray:
 
 
 def random_mixed_circuit(n: int, num_ops: int, seed: int, *, unitaries: bool = True, measures: bool = False,
-                         max_unitary: int = 5) -> W.Circuit:
-    """Every named gate kind (arity permitting), random angles and operand
-    orders, k-qubit unitaries with shuffled operands, optional mid-circuit
-    measurement barriers."""
+                         max_unitary: int = 5, max_arity: int = 3) -> W.Circuit:
+    """Every named gate kind (arity permitting, at most max_arity operands),
+    random angles and operand orders, k-qubit unitaries with shuffled operands,
+    optional mid-circuit measurement barriers."""
     rng = np.random.default_rng(seed)
     c = W.Circuit(n, num_clbits=n)
     c.global_phase = float(rng.uniform(-np.pi, np.pi))
-    kinds = [op for op, a in W.GATE_ARITY.items() if a <= n]
+    kinds = [op for op, a in W.GATE_ARITY.items() if a <= min(n, max_arity)]
     for _ in range(num_ops):
         roll = rng.random()
         if unitaries and roll < 0.08:
