@@ -199,7 +199,7 @@ class DeviceState:
         out = np.zeros(2 * max(len(terms), 1), np.float64)
         check(lib().mh_expectation(self.ptr, self.n, self.precision, _ptr(xs, _u64p), _ptr(zs, _u64p), len(terms),
                                    out.ctypes.data_as(_f64p)))
-        return (out[0::2] + 1j * out[1::2])[:len(terms)]
+        return out.view(np.complex128)[:len(terms)].copy()
 
     def sample(self, sorted_uniforms) -> np.ndarray:
         u = np.ascontiguousarray(sorted_uniforms, np.float64)
@@ -307,7 +307,7 @@ class HipDenseEngine:
         check(lib().mh_engine_execute(ctypes.byref(req), ctypes.byref(res)))
         counts = {int(outc[i]): int(cnt[i]) for i in range(res.num_outcomes)} if shots else None
         return Result(statevector=sv, counts=counts,
-                      expectation=(ex[0::2] + 1j * ex[1::2])[:len(terms)] if terms else None,
+                      expectation=ex.view(np.complex128)[:len(terms)].copy() if terms else None,
                       pass_ms=pms[:min(res.num_passes, max_passes)] if record_pass_times else None,
                       num_blocks=res.num_blocks, num_passes=res.num_passes, fused_blocks=res.fused_blocks,
                       gates_absorbed=res.gates_absorbed, actual_bytes=res.actual_bytes,

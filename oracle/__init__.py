@@ -225,7 +225,7 @@ class Oracle(_Api):
                              idx.ctypes.data_as(_u64p) if idx is not None else None,
                              ex.ctypes.data_as(_f64p) if ex is not None else None)
         del keep
-        e = ex[0::2] + 1j * ex[1::2] if ex is not None else np.zeros(0, np.complex128)
+        e = ex.view(np.complex128).copy() if ex is not None else np.zeros(0, np.complex128)
         return sv, (idx[:shots] if idx is not None else np.zeros(0, np.uint64)), e[:len(terms)]
 
 
@@ -312,7 +312,7 @@ class Reference(_Api):
                                     ex.ctypes.data_as(_f64p) if ex is not None else None)
         del keep
         counts = {int(outc[i]): int(cnt[i]) for i in range(nout.value)} if shots else {}
-        e = ex[0::2] + 1j * ex[1::2] if ex is not None else np.zeros(0, np.complex128)
+        e = ex.view(np.complex128).copy() if ex is not None else np.zeros(0, np.complex128)
         return sv, counts, e[:len(terms)], secs
 
 

@@ -61,5 +61,6 @@ def circuit_from_spec(spec):
 
 
 def cpx(pairs):
-    a = np.asarray(pairs, np.float64)
-    return a[:, 0] + 1j * a[:, 1]
+    """[(re, im), ...] as complex128, keeping the sign of every zero."""
+    a = np.ascontiguousarray(np.asarray(pairs, np.float64).reshape(-1, 2))
+    return a.view(np.complex128).reshape(-1)
