@@ -260,7 +260,7 @@ int mh_calibrate(mh_calibration* out);
  * reference's, expectation terms as mh_expectation's. -------------------- */
 typedef struct mh_batch_request {
     uint32_t num_circuits;
-    const mh_circuit* circuits;    /* each with at least 10 (complex128) / 11 (complex64) qubits */
+    const mh_circuit* circuits;    /* each with at least 10 qubits */
     int32_t precision;
     int32_t fusion;                /* mh_fusion */
     uint32_t fusion_width;

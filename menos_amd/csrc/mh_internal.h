@@ -114,8 +114,7 @@ struct PassStats {
 void apply_blocks(void* state, uint32_t n, int32_t precision, const std::vector<Block>& blocks,
                   uint32_t window_qubits, bool time_passes, PassStats* stats, uint32_t max_cost);
 
-// The default window: one wavefront's registers (10 qubits at complex128, 11
-// at complex64).
+// The default window: one wavefront's registers (10 qubits).
 uint32_t window_default(int32_t precision) noexcept;
 
 // Number of HBM passes the schedule makes for these blocks.

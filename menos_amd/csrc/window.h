@@ -14,7 +14,7 @@ namespace mh {
 constexpr uint32_t kDefaultPassCost = 160;
 
 // Window qubits of one pass: the 6 lane bits of a wavefront plus the
-// register bits of a lane (4 at complex128, 5 at complex64).
+// register bits of a lane (4).
 uint32_t window_width(int32_t precision) noexcept;
 
 struct WindowPass {

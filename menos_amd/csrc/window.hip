@@ -2,8 +2,8 @@
 //
 // Each wavefront takes one window of W qubits (the other qubits fixed by the
 // wave's index) and holds its 2^W amplitudes in registers: lane bit i of the
-// wave's 64 lanes and register bit j of a lane's 2^B amplitudes (B = 4 at
-// complex128, 5 at complex64; W = 6 + B) each stand for one window qubit. A
+// wave's 64 lanes and register bit j of a lane's 2^B amplitudes (B = 4, so
+// W = 10) each stand for one window qubit. A
 // dense block acts inside every lane once its operands sit on register bits.
 // Moving a qubit between a lane bit and a register bit is an exchange between
 // partner lanes done in registers (v_permlane32_swap / v_permlane16_swap for
@@ -58,7 +58,7 @@ template <> struct Shape<double> {
     using V = double2;
 };
 template <> struct Shape<float> {
-    static constexpr uint32_t kRegBits = 5, kLow = 4;
+    static constexpr uint32_t kRegBits = 4, kLow = 4;
     using V = float2;
 };
 
