@@ -121,7 +121,7 @@ def main() -> int:
     ap.add_argument("--window", type=int, default=0)
     ap.add_argument("--max-pass-cost", type=int, default=0)
     ap.add_argument("--cpu-qubits", type=int, default=28)
-    ap.add_argument("--cpu-layers", type=int, default=6)
+    ap.add_argument("--cpu-layers", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-circuits", type=int, default=4096)
     ap.add_argument("--per-circuit-sample", type=int, default=256)
