@@ -108,6 +108,47 @@ def batched(args, H, W) -> dict:
     return out
 
 
+def complex64_report(args, H, W) -> dict:
+    """The complex64 form: per-pass HBM rate of the 16-layer random circuit at
+    34 qubits (128 GiB), and at 33 qubits its error against the complex128
+    run of the same gates: the largest amplitude difference and the largest
+    <Z_i Z_i+1> difference."""
+    import numpy as np
+    out = {}
+    n34 = args.c64_qubits
+    c = W.random_circuit(n34, args.layers, 2025)
+    eng = H.HipDenseEngine()
+    r = eng.execute(c, precision=H.C64, statevector=False, record_pass_times=True)
+    ms = r.pass_ms.astype(np.float64)
+    out["c64_qubits"] = n34
+    out["c64_passes"] = int(r.num_passes)
+    out["c64_gate_seconds"] = round(float(r.gate_seconds), 4)
+    out["c64_pass_GBps"] = round(2 * 8 * (1 << n34) * len(ms) / (ms.sum() * 1e-3) / 1e9, 1)
+    log(f"complex64 {n34} qubits: {out['c64_passes']} passes in {r.gate_seconds:.3f} s, "
+        f"{out['c64_pass_GBps']} GB/s per pass")
+    n = args.qubits
+    c = W.random_circuit(n, args.layers, 2025)
+    blocks = [(H.gate_matrix(op, params), list(qs)) for op, qs, params in c.ops]
+    states = {}
+    for prec in (H.C128, H.C64):
+        d = H.DeviceState(n, prec)
+        d.init()
+        d.apply_blocks(blocks)
+        states[prec] = d
+    terms = W.zz_neighbour_terms(n)
+    e128 = states[H.C128].expectation(terms)
+    e64 = states[H.C64].expectation(terms)
+    worst = states[H.C128].max_abs_diff(states[H.C64])
+    for d in states.values():
+        d.free()
+    out["error_qubits"] = n
+    out["max_amplitude_error"] = worst
+    out["max_zz_error"] = float(np.max(np.abs(e128 - e64)))
+    log(f"complex64 vs complex128 at {n} qubits: max amplitude error {worst:.3g}, "
+        f"max <ZZ> error {out['max_zz_error']:.3g}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +167,9 @@ def main() -> int:
     ap.add_argument("--batch-circuits", type=int, default=4096)
     ap.add_argument("--per-circuit-sample", type=int, default=256)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--c64-report", action="store_true",
+                    help="also measure the complex64 form at --c64-qubits and its error against complex128")
+    ap.add_argument("--c64-qubits", type=int, default=34)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run the library's calibration (its streaming copy of 2 GiB each way is the "
                          "reference for scaling HBM traffic counters)")
@@ -215,6 +259,8 @@ def main() -> int:
     }
     if not args.no_batch:
         out["batched"] = batched(args, H, W)
+    if args.c64_report:
+        out["complex64"] = complex64_report(args, H, W)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline: {args.cpu_qubits} qubits, {args.cpu_layers} layers, {threads} threads")

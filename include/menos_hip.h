@@ -133,6 +133,12 @@ int mh_copy_from_host(void* state, uint32_t num_qubits, int32_t precision, const
 /* FNV-1a 64 over the state's bytes, computed on the host after a copy-back
  * in chunks (the reference's digest, tests/memory/memory_fixtures.cpp:599-645). */
 int mh_state_digest(const void* state, uint32_t num_qubits, int32_t precision, uint64_t* out);
+/* The largest |a_i - b_i| (complex modulus in FP64) between two device states
+ * of the same size, of either precision: the complex64 form's amplitude error
+ * against a complex128 run. No counterpart in the reference (its CUDA engine
+ * has one precision); used by the error reports. */
+int mh_state_max_abs_diff(const void* a, int32_t precision_a, const void* b, int32_t precision_b,
+                          uint32_t num_qubits, double* out);
 
 /* ---- The engine. Replaces CudaDenseEngine::{estimate, execute}
  * (reference src/engines/dense_cuda/cuda_dense_engine.cpp:182-412) behind the

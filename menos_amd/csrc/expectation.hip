@@ -360,6 +360,44 @@ void expectation_batch(int32_t precision, const std::vector<ExpectationItem>& it
         expectation_batch_typed<float2>(items, out);
 }
 
+namespace {
+
+// Largest |a_i - b_i| (complex modulus, FP64) over two states of any precisions:
+// one partial per workgroup.
+template <typename VA, typename VB>
+__global__ void __launch_bounds__(kThreads) max_diff_kernel(const VA* a, const VB* b, uint64_t size, double* partials) {
+    __shared__ double red[kThreads];
+    double m = 0.0;
+    const uint64_t stride = uint64_t(gridDim.x) * kThreads;
+    for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < size; i += stride) {
+        const double dr = static_cast<double>(a[i].x) - static_cast<double>(b[i].x);
+        const double di = static_cast<double>(a[i].y) - static_cast<double>(b[i].y);
+        m = fmax(m, sqrt(dr * dr + di * di));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+template <typename VA, typename VB>
+double max_diff(const void* a, const void* b, uint64_t size) {
+    Context& ctx = context();
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(4096, std::max<uint64_t>(1, size / kThreads)));
+    DeviceBuffer partials(grid * sizeof(double));
+    hipLaunchKernelGGL((max_diff_kernel<VA, VB>), dim3(grid), dim3(kThreads), 0, ctx.stream, static_cast<const VA*>(a),
+                       static_cast<const VB*>(b), size, partials.as<double>());
+    MH_HIP(hipGetLastError());
+    std::vector<double> host(grid);
+    download(host.data(), partials.get(), grid * sizeof(double));
+    return *std::max_element(host.begin(), host.end());
+}
+
+}  // namespace
+
 }  // namespace mh
 
 using namespace mh;
@@ -368,5 +406,24 @@ extern "C" int mh_expectation(const void* state, uint32_t num_qubits, int32_t pr
                               const uint64_t* z_masks, uint32_t num_terms, double* out) {
     return guarded([&] {
         expectation_terms(state, num_qubits, precision, x_masks, z_masks, num_terms, reinterpret_cast<cplx*>(out));
+    });
+}
+
+extern "C" int mh_state_max_abs_diff(const void* a, int32_t precision_a, const void* b, int32_t precision_b,
+                                     uint32_t num_qubits, double* out) {
+    return guarded([&] {
+        amp_bytes(precision_a);
+        amp_bytes(precision_b);
+        if (num_qubits > 40) fail(MH_ERR_INVALID, "num_qubits above 40");
+        if (out == nullptr) fail(MH_ERR_INVALID, "null output");
+        const uint64_t size = uint64_t{1} << num_qubits;
+        if (precision_a == MH_C128 && precision_b == MH_C128)
+            *out = max_diff<double2, double2>(a, b, size);
+        else if (precision_a == MH_C128)
+            *out = max_diff<double2, float2>(a, b, size);
+        else if (precision_b == MH_C128)
+            *out = max_diff<float2, double2>(a, b, size);
+        else
+            *out = max_diff<float2, float2>(a, b, size);
     });
 }

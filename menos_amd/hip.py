@@ -26,7 +26,7 @@ EXPORTS = (
     "mh_pool_reusable_bytes", "mh_pool_trim", "mh_state_init", "mh_apply_matrix", "mh_apply_matrix_generic",
     "mh_apply_blocks", "mh_expectation", "mh_sample", "mh_sample_unsorted", "mh_copy_to_host",
     "mh_copy_from_host", "mh_state_digest", "mh_engine_estimate", "mh_engine_execute", "mh_plan_fusion",
-    "mh_gate_matrix", "mh_calibrate", "mh_batch_execute",
+    "mh_gate_matrix", "mh_calibrate", "mh_batch_execute", "mh_state_max_abs_diff",
 )
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -136,6 +136,13 @@ class DeviceState:
         p = ctypes.c_void_p()
         check(lib().mh_pool_alloc(ctypes.c_uint64(self.nbytes), ctypes.byref(p)))
         self.ptr = p
+
+    def max_abs_diff(self, other: "DeviceState") -> float:
+        """Largest |a_i - b_i| against another device state of the same size."""
+        out = ctypes.c_double()
+        check(lib().mh_state_max_abs_diff(self.ptr, self.precision, other.ptr, other.precision, self.n,
+                                          ctypes.byref(out)))
+        return out.value
 
     def free(self):
         if self.ptr:

@@ -281,3 +281,24 @@ def test_invalid_operands_are_refused(hip):
     with pytest.raises(hip.HipError, match="repeated"):
         d.apply_matrix(np.eye(4), [1, 1])
     d.free()
+
+
+def test_state_max_abs_diff(hip):
+    """mh_state_max_abs_diff: zero against itself, the planted difference
+    otherwise, across precisions."""
+    n = 12
+    s = random_state(n, 4)
+    a = hip.DeviceState(n)
+    a.upload(s)
+    assert a.max_abs_diff(a) == 0.0
+    t = s.copy()
+    t[777] += 3e-3 + 4e-3j
+    b = hip.DeviceState(n)
+    b.upload(t)
+    assert abs(a.max_abs_diff(b) - abs(t[777] - s[777])) < 1e-15
+    c = hip.DeviceState(n, hip.C64)
+    c.upload(s.astype(np.complex64))
+    want = float(np.max(np.abs(s - s.astype(np.complex64).astype(np.complex128))))
+    assert abs(a.max_abs_diff(c) - want) < 1e-15
+    for d in (a, b, c):
+        d.free()
