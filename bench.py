@@ -108,6 +108,68 @@ def batched(args, H, W) -> dict:
     return out
 
 
+def served(args, H, W) -> dict:
+    """The served workload through the job server (mh_server_*): `args.serve_jobs`
+    jobs (n log-uniform on 12..30; 70% estimator jobs, an ansatz of depth 4..12
+    with 16 observables of 1..4 Pauli terms, at `args.serve_estimator_precision`;
+    30% sampler jobs, 20 layers of U3 + CZ brick with 1000 shots, complex128;
+    40% parameter-only updates of an earlier job's circuit), under a 256 GB
+    ledger budget. First every job is submitted at once: capacity = jobs / wall
+    time. Then the same jobs arrive as a Poisson process at 0.8 x that capacity
+    (unless `args.serve_rate` is set): circuits/s, p50 and p99 of
+    submit -> result latency."""
+    import numpy as np
+    prec = H.C64 if args.serve_estimator_precision == "c64" else H.C128
+    t0 = time.perf_counter()
+    specs = W.serve_workload(args.serve_jobs, 1.0, 2027)
+    gaps = np.array([s.arrival for s in specs])  # arrivals at rate 1/s, rescaled below
+
+    def jobs_of():
+        return [H.Job(H.JOB_ESTIMATOR if s.kind == "estimator" else H.JOB_SAMPLER, s.circuit,
+                      precision=prec if s.kind == "estimator" else H.C128, observables=s.observables,
+                      shots=s.shots, seed=s.seed) for s in specs]
+
+    jobs = jobs_of()
+    log(f"served: {len(jobs)} jobs built in {time.perf_counter() - t0:.1f} s")
+    out = {"workload": f"{len(jobs)} jobs, n log-uniform 12..30; 70% estimator (ansatz depth 4..12, 16 observables, "
+                       f"{args.serve_estimator_precision}), 30% sampler (U3 + CZ brick, 20 layers, 1000 shots, "
+                       "c128); 40% parameter-only updates; 256 GB ledger budget",
+           "jobs": len(jobs),
+           "estimators": sum(s.kind == "estimator" for s in specs),
+           "updates": sum(s.update for s in specs)}
+    with H.Server(budget_bytes=256_000_000_000) as srv:
+        w0 = time.perf_counter()
+        res = srv.replay(jobs, np.zeros(len(jobs)))
+        wall = time.perf_counter() - w0
+        st = srv.stats()
+    failed = [r for r in res if r.status != 0]
+    if failed:
+        raise RuntimeError(f"{len(failed)} served jobs failed: {failed[0].error}")
+    cap = len(jobs) / wall
+    out["capacity"] = {"circuits_per_s": round(cap, 2), "seconds": round(wall, 3), "rounds": st.rounds,
+                       "plan_cache_hits": st.plan_cache_hits, "plan_cache_misses": st.plan_cache_misses,
+                       "peak_round_GB": round(st.peak_round_bytes / 1e9, 2)}
+    log(f"served capacity: {out['capacity']}")
+    rate = args.serve_rate or 0.8 * cap
+    arrivals = gaps / rate
+    jobs = jobs_of()
+    with H.Server(budget_bytes=256_000_000_000) as srv:
+        w0 = time.perf_counter()
+        res = srv.replay(jobs, arrivals)
+        wall = time.perf_counter() - w0
+        st = srv.stats()
+    failed = [r for r in res if r.status != 0]
+    if failed:
+        raise RuntimeError(f"{len(failed)} served jobs failed: {failed[0].error}")
+    lat = np.array([r.latency for r in res]) * 1e3
+    out["poisson"] = {"rate_per_s": round(rate, 2), "circuits_per_s": round(len(jobs) / wall, 2),
+                      "seconds": round(wall, 3), "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                      "p99_ms": round(float(np.percentile(lat, 99)), 2), "rounds": st.rounds,
+                      "plan_cache_hits": st.plan_cache_hits, "busy_seconds": round(st.busy_seconds, 3)}
+    log(f"served Poisson: {out['poisson']}")
+    return out
+
+
 def complex64_report(args, H, W) -> dict:
     """The complex64 form: per-pass HBM rate of the 16-layer random circuit at
     34 qubits (128 GiB), and at 33 qubits its error against the complex128
@@ -167,6 +229,10 @@ def main() -> int:
     ap.add_argument("--batch-circuits", type=int, default=4096)
     ap.add_argument("--per-circuit-sample", type=int, default=256)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=1000)
+    ap.add_argument("--serve-rate", type=float, default=0.0, help="arrivals per second; 0: 0.8 x measured capacity")
+    ap.add_argument("--serve-estimator-precision", choices=["c64", "c128"], default="c64")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--c64-report", action="store_true",
                     help="also measure the complex64 form at --c64-qubits and its error against complex128")
     ap.add_argument("--c64-qubits", type=int, default=34)
@@ -259,6 +325,8 @@ def main() -> int:
     }
     if not args.no_batch:
         out["batched"] = batched(args, H, W)
+    if not args.no_serve:
+        out["served"] = served(args, H, W)
     if args.c64_report:
         out["complex64"] = complex64_report(args, H, W)
     if not args.no_cpu:

@@ -292,6 +292,107 @@ typedef struct mh_batch_result {
 
 int mh_batch_execute(const mh_batch_request* request, mh_batch_result* result);
 
+/* ---- Serving: a resident job server. Replaces the per-pub sequential
+ * `run` loops of the reference's Python primitives (MENOSEstimator._run_pub,
+ * python/menos/qiskit/estimator.py:91-146: one run per observable;
+ * MENOSSampler._run_pub, sampler.py:93-129) for a stream of jobs. Jobs are
+ * queued as they are submitted; a worker thread takes the queued jobs in
+ * arrival order while their states fit the device budget (the ledger
+ * admission, reference device_buffer.cpp:22-27), runs them as one batched
+ * round (mh_batch_execute's kernels) and completes them. A job whose state
+ * alone exceeds the budget completes with MH_ERR_RESOURCE.
+ *
+ * Fusion plans are cached by circuit structure (opcodes, qubits, parameter
+ * counts, unitary entries), so a parameter-only update of a circuit seen
+ * before reuses its plan, as the reference's plan cache does
+ * (plan_job_cached / reparameterize_direct_plan, src/plan/plan_cache.cpp:233-276,
+ * planner.cpp:1601-1652). Results are the single-circuit engine's bit for bit:
+ * raw terms as mh_engine_execute's, counts as its counts for the same seed,
+ * and each observable's value combined from its raw terms in term order from
+ * (0, 0) as the reference's combine_expectation does
+ * (src/runtime/combine_expectation.cpp:7-25). --------------------------- */
+typedef struct mh_server mh_server;
+
+typedef struct mh_server_options {
+    uint64_t budget_bytes;          /* device bytes of one round's states; 0: 256 GB, capped to what the device has */
+    uint32_t max_round_jobs;        /* 0: no cap */
+    uint32_t host_threads;          /* planning threads; 0: the hardware's, at most 16 */
+    int32_t fusion;                 /* mh_fusion */
+    uint32_t fusion_width;
+    uint32_t plan_cache_capacity;   /* cached fusion plans; 0: 256 */
+} mh_server_options;
+
+typedef enum mh_job_kind { MH_JOB_ESTIMATOR = 0, MH_JOB_SAMPLER = 1 } mh_job_kind;
+
+typedef struct mh_job {
+    int32_t kind;                   /* mh_job_kind */
+    int32_t precision;              /* a sampler job must be MH_C128 */
+    const mh_circuit* circuit;      /* 10..30 qubits; copied at submission */
+    double global_phase;
+    /* Estimator: observables, each a sum of Pauli terms with complex coefficients. */
+    uint32_t num_observables;
+    const uint32_t* observable_offsets; /* num_observables + 1 offsets into the term arrays */
+    const uint64_t* x_masks;
+    const uint64_t* z_masks;
+    const double* coefficients;     /* (re, im) per term */
+    /* Sampler: shots drawn with `seed` (the pub's SplitMix64 seed, mh_derive_seeds). */
+    uint64_t shots;
+    uint64_t seed;
+    uint32_t num_targets;
+    const uint32_t* target_qubits;  /* measure targets (qubit -> clbit) */
+    const uint32_t* target_clbits;
+    uint32_t bitstring_width;       /* <= 64 */
+} mh_job;
+
+typedef struct mh_job_result {
+    /* Caller-owned outputs, sized from the job; any may be null. */
+    double* observable_values;      /* 2 doubles (re, im) per observable */
+    double* raw_terms;              /* 2 doubles per term */
+    uint64_t* outcomes;             /* capacity `shots`: distinct outcomes, ascending */
+    uint64_t* counts;
+    /* Filled by the server. */
+    int32_t status;                 /* MH_OK or the job's failure */
+    char error[256];
+    uint64_t num_outcomes;
+    uint32_t round;                 /* the round that ran it (1-based) */
+    int32_t plan_cache_hit;
+    double submitted_seconds;       /* from the server's start */
+    double started_seconds;         /* its round's start */
+    double completed_seconds;       /* its results ready */
+} mh_job_result;
+
+typedef struct mh_server_stats {
+    uint64_t jobs_completed;
+    uint64_t jobs_failed;
+    uint32_t rounds;
+    uint64_t plan_cache_hits;
+    uint64_t plan_cache_misses;
+    uint64_t peak_round_bytes;      /* the ledger's peak: the largest round's states */
+    uint64_t budget_bytes;          /* the budget in force */
+    double busy_seconds;            /* wall time inside rounds */
+} mh_server_stats;
+
+int mh_server_create(const mh_server_options* options, mh_server** out);
+/* Copies the job (circuit, terms, targets) and returns at once with its ticket. */
+int mh_server_submit(mh_server* server, const mh_job* job, uint64_t* ticket);
+/* Blocks until the ticket's job is complete and writes its result; a ticket
+ * is waited for once. Returns MH_OK when the result was written (the job's
+ * own outcome is result->status). */
+int mh_server_wait(mh_server* server, uint64_t ticket, mh_job_result* result);
+/* Submits job i at arrival_seconds[i] (ascending, from the call's start) on a
+ * submitter thread, then waits for every job. For arrival-process benchmarks. */
+int mh_server_replay(mh_server* server, uint32_t count, const mh_job* jobs, const double* arrival_seconds,
+                     mh_job_result* results);
+int mh_server_stats_get(mh_server* server, mh_server_stats* out);
+/* Completes the queued jobs, stops the worker and frees the server. */
+int mh_server_destroy(mh_server* server);
+
+/* The first `count` outputs of SplitMix64 seeded with `seed`: the per-circuit
+ * seeds of run_batch and the per-pub seeds of the sampler (reference
+ * src/common/splitmix64.h:15-28, src/api/batch.cpp:18-24,
+ * src/python/bindings.cpp:581-595). */
+int mh_derive_seeds(uint64_t seed, uint32_t count, uint64_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,53 @@
+// Batched rounds (batch.hip): many planned circuits executed together.
+#pragma once
+
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <utility>
+#include <vector>
+
+#include "mh_internal.h"
+
+namespace mh {
+
+// One planned circuit of a round.
+struct RoundItem {
+    uint32_t n = 0;
+    const std::vector<Block>* blocks = nullptr;
+    double global_phase = 0.0;
+    // Expectation terms: results land at terms_out[2 * (term_out + t)].
+    uint32_t num_terms = 0;
+    const uint64_t* x = nullptr;
+    const uint64_t* z = nullptr;
+    uint32_t term_out = 0;
+    // Sampling (shots > 0): counts keyed by the clbit string.
+    uint64_t shots = 0;
+    uint64_t seed = 0;
+    const std::vector<std::pair<uint32_t, uint32_t>>* targets = nullptr;
+    std::map<uint64_t, uint64_t>* counts = nullptr;
+    // Host destination of the final amplitudes, or null.
+    void* statevector = nullptr;
+};
+
+struct RoundStats {
+    uint32_t launches = 0;
+};
+
+// Device bytes one state of a round takes in the round's arena.
+uint64_t round_state_bytes(uint32_t n, int32_t precision);
+
+// Runs one round: every item's state in one pooled arena, initialised to its
+// global phase, pass p of every item in one launch, all expectation terms in
+// two launches, then each sampling item's draw and statevector copy.
+// Synchronises. Every item needs window_width(precision) to 30 qubits.
+// `arena` (device memory of at least the items' round_state_bytes in total)
+// holds the states; null takes an arena from the pool for the call.
+void execute_round(int32_t precision, const std::vector<RoundItem>& items, uint32_t threads, double* terms_out,
+                   RoundStats* stats, void* arena = nullptr);
+
+// Runs body(i) for i in [0, count) on up to `threads` host threads; rethrows
+// the first exception.
+void parallel_for(std::size_t count, uint32_t threads, const std::function<void(std::size_t)>& body);
+
+}  // namespace mh

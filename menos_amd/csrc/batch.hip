@@ -24,6 +24,8 @@
 #include <mutex>
 #include <thread>
 
+#include "batch.h"
+#include "engine.hpp"
 #include "gate_table.h"
 #include "mh_internal.h"
 #include "window.h"
@@ -49,8 +51,11 @@ struct Planned {
     std::vector<Block> blocks;
 };
 
-template <typename F>
-void parallel_for(std::size_t count, uint32_t threads, F&& body) {
+constexpr uint64_t kAlign = 256;
+
+}  // namespace
+
+void parallel_for(std::size_t count, uint32_t threads, const std::function<void(std::size_t)>& body) {
     std::atomic<std::size_t> next{0};
     std::exception_ptr error;
     std::mutex m;
@@ -70,7 +75,90 @@ void parallel_for(std::size_t count, uint32_t threads, F&& body) {
     if (error) std::rethrow_exception(error);
 }
 
-}  // namespace
+uint64_t round_state_bytes(uint32_t n, int32_t precision) {
+    return ((uint64_t{1} << n) * amp_bytes(precision) + kAlign - 1) / kAlign * kAlign;
+}
+
+void execute_round(int32_t precision, const std::vector<RoundItem>& items, uint32_t threads, double* terms_out,
+                   RoundStats* stats, void* arena_in) {
+    const std::size_t amp = amp_bytes(precision);
+    const uint32_t width = window_width(precision);
+    if (items.empty()) return;
+    uint64_t bytes = 0;
+    for (const RoundItem& it : items) {
+        if (it.n < width || it.n > 30)
+            fail(MH_ERR_INVALID, "a batched circuit needs " + std::to_string(width) + " to 30 qubits");
+        if (it.num_terms > 0 && terms_out == nullptr) fail(MH_ERR_INVALID, "terms without an output");
+        bytes += round_state_bytes(it.n, precision);
+    }
+    Context& ctx = context();
+    DeviceBuffer own(arena_in ? 0 : bytes);
+    unsigned char* const arena = static_cast<unsigned char*>(arena_in ? arena_in : own.get());
+    const std::size_t count = items.size();
+    std::vector<uint64_t> offsets(count);  // amplitudes from the arena start
+    std::vector<double> phases(2 * count);
+    std::vector<BatchItem> passes(count);
+    std::vector<ExpectationItem> exp_items;
+    uint64_t at = 0;
+    for (std::size_t k = 0; k < count; ++k) {
+        const RoundItem& it = items[k];
+        offsets[k] = at / amp;
+        passes[k].state = arena + at;
+        passes[k].n = it.n;
+        passes[k].blocks = it.blocks;
+        const std::complex<double> ph = host_phase(it.global_phase);
+        phases[2 * k] = ph.real();
+        phases[2 * k + 1] = ph.imag();
+        if (it.num_terms > 0) {
+            ExpectationItem e;
+            e.state = passes[k].state;
+            e.n = it.n;
+            e.terms = it.num_terms;
+            e.x = it.x;
+            e.z = it.z;
+            e.out_index = it.term_out;
+            exp_items.push_back(e);
+        }
+        at += round_state_bytes(it.n, precision);
+    }
+    // Initialise: zeros, then each state's first amplitude.
+    MH_HIP(hipMemsetAsync(arena, 0, bytes, ctx.stream));
+    DeviceBuffer doffsets(offsets.size() * sizeof(uint64_t)), dphases(phases.size() * sizeof(double));
+    upload(doffsets.get(), offsets.data(), offsets.size() * sizeof(uint64_t));
+    upload(dphases.get(), phases.data(), phases.size() * sizeof(double));
+    const uint32_t n_here = static_cast<uint32_t>(count);
+    const dim3 grid((n_here + kThreads - 1) / kThreads), block(kThreads);
+    if (precision == MH_C128)
+        hipLaunchKernelGGL(first_amplitude_kernel<double2>, grid, block, 0, ctx.stream, reinterpret_cast<double2*>(arena),
+                           static_cast<const uint64_t*>(doffsets.as<uint64_t>()),
+                           static_cast<const double*>(dphases.as<double>()), n_here);
+    else
+        hipLaunchKernelGGL(first_amplitude_kernel<float2>, grid, block, 0, ctx.stream, reinterpret_cast<float2*>(arena),
+                           static_cast<const uint64_t*>(doffsets.as<uint64_t>()),
+                           static_cast<const double*>(dphases.as<double>()), n_here);
+    MH_HIP(hipGetLastError());
+    // Apply, reduce, sample, copy back.
+    BatchStats bs;
+    run_window_passes_batch(precision, passes, threads, &bs);
+    uint32_t launches = 2 + bs.launches;
+    if (!exp_items.empty()) {
+        expectation_batch(precision, exp_items, terms_out);
+        launches += 2;
+    }
+    for (std::size_t k = 0; k < count; ++k) {
+        const RoundItem& it = items[k];
+        if (it.shots > 0) {
+            static const std::vector<std::pair<uint32_t, uint32_t>> kNone;
+            *it.counts = sample_counts(passes[k].state, it.n, precision, it.shots, it.seed,
+                                       it.targets ? *it.targets : kNone);
+            launches += 7;
+        }
+        if (it.statevector) download(it.statevector, passes[k].state, (uint64_t{1} << it.n) * amp);
+    }
+    synchronize();
+    if (stats) stats->launches += launches;
+}
+
 }  // namespace mh
 
 using namespace mh;
@@ -108,87 +196,43 @@ extern "C" int mh_batch_execute(const mh_batch_request* req, mh_batch_result* re
         });
         res->plan_seconds = std::chrono::duration<double>(clock::now() - t0).count();
 
-        // 2. Rounds.
+        // 2. Rounds of states that fit round_bytes; 3. each round executed.
         uint64_t round_bytes = req->round_bytes;
         if (round_bytes == 0) {
             size_t free_b = 0, total_b = 0;
             MH_HIP(hipMemGetInfo(&free_b, &total_b));
             round_bytes = (static_cast<uint64_t>(free_b) + pool_reusable_bytes()) / 2;
         }
-        constexpr uint64_t kAlign = 256;
-        Context& ctx = context();
         uint64_t sv_at = 0;
-        uint32_t rounds = 0, launches = 0;
+        uint32_t rounds = 0;
+        RoundStats rs;
         double device_seconds = 0;
         for (uint32_t first = 0; first < count;) {
             uint32_t last = first;
             uint64_t bytes = 0;
             while (last < count) {
-                const uint64_t b = ((uint64_t{1} << plans[last].n) * amp + kAlign - 1) / kAlign * kAlign;
+                const uint64_t b = round_state_bytes(plans[last].n, req->precision);
                 if (last > first && bytes + b > round_bytes) break;
                 bytes += b;
                 ++last;
             }
             const auto r0 = clock::now();
-            DeviceBuffer arena(bytes);
-            std::vector<uint64_t> offsets(last - first);  // amplitudes from the arena start
-            std::vector<double> phases(2 * (last - first));
-            std::vector<BatchItem> items(last - first);
-            std::vector<ExpectationItem> exp_items;
-            uint64_t at = 0;
+            std::vector<RoundItem> items(last - first);
             for (uint32_t i = first; i < last; ++i) {
-                const uint32_t k = i - first;
-                offsets[k] = at / amp;
-                items[k].state = static_cast<unsigned char*>(arena.get()) + at;
-                items[k].n = plans[i].n;
-                items[k].blocks = &plans[i].blocks;
-                const std::complex<double> ph = host_phase(req->global_phases ? req->global_phases[i] : 0.0);
-                phases[2 * k] = ph.real();
-                phases[2 * k + 1] = ph.imag();
-                const uint32_t t_begin = req->term_offsets[i], t_end = req->term_offsets[i + 1];
-                if (t_end > t_begin) {
-                    ExpectationItem e;
-                    e.state = items[k].state;
-                    e.n = plans[i].n;
-                    e.terms = t_end - t_begin;
-                    e.x = req->x_masks + t_begin;
-                    e.z = req->z_masks + t_begin;
-                    e.out_index = t_begin;
-                    exp_items.push_back(e);
-                }
-                at += ((uint64_t{1} << plans[i].n) * amp + kAlign - 1) / kAlign * kAlign;
-            }
-            // 3. Initialise, apply, reduce.
-            MH_HIP(hipMemsetAsync(arena.get(), 0, bytes, ctx.stream));
-            DeviceBuffer doffsets(offsets.size() * sizeof(uint64_t)), dphases(phases.size() * sizeof(double));
-            upload(doffsets.get(), offsets.data(), offsets.size() * sizeof(uint64_t));
-            upload(dphases.get(), phases.data(), phases.size() * sizeof(double));
-            const uint32_t n_here = last - first;
-            const dim3 grid((n_here + kThreads - 1) / kThreads), block(kThreads);
-            if (req->precision == MH_C128)
-                hipLaunchKernelGGL(first_amplitude_kernel<double2>, grid, block, 0, ctx.stream, arena.as<double2>(),
-                                   static_cast<const uint64_t*>(doffsets.as<uint64_t>()),
-                                   static_cast<const double*>(dphases.as<double>()), n_here);
-            else
-                hipLaunchKernelGGL(first_amplitude_kernel<float2>, grid, block, 0, ctx.stream, arena.as<float2>(),
-                                   static_cast<const uint64_t*>(doffsets.as<uint64_t>()),
-                                   static_cast<const double*>(dphases.as<double>()), n_here);
-            MH_HIP(hipGetLastError());
-            BatchStats stats;
-            run_window_passes_batch(req->precision, items, threads, &stats);
-            launches += 2 + stats.launches;
-            if (!exp_items.empty()) {
-                expectation_batch(req->precision, exp_items, res->expectation_terms);
-                launches += 2;
-            }
-            if (req->want_statevectors) {
-                for (uint32_t i = first; i < last; ++i) {
-                    const uint64_t sb = (uint64_t{1} << plans[i].n) * amp;
-                    download(static_cast<unsigned char*>(res->statevectors) + sv_at, items[i - first].state, sb);
-                    sv_at += sb;
+                RoundItem& it = items[i - first];
+                it.n = plans[i].n;
+                it.blocks = &plans[i].blocks;
+                it.global_phase = req->global_phases ? req->global_phases[i] : 0.0;
+                it.num_terms = req->term_offsets[i + 1] - req->term_offsets[i];
+                it.x = req->x_masks + req->term_offsets[i];
+                it.z = req->z_masks + req->term_offsets[i];
+                it.term_out = req->term_offsets[i];
+                if (req->want_statevectors) {
+                    it.statevector = static_cast<unsigned char*>(res->statevectors) + sv_at;
+                    sv_at += (uint64_t{1} << plans[i].n) * amp;
                 }
             }
-            synchronize();
+            execute_round(req->precision, items, threads, res->expectation_terms, &rs);
             device_seconds += std::chrono::duration<double>(clock::now() - r0).count();
             const double done = std::chrono::duration<double>(clock::now() - t0).count();
             if (res->completion_seconds)
@@ -196,6 +240,7 @@ extern "C" int mh_batch_execute(const mh_batch_request* req, mh_batch_result* re
             ++rounds;
             first = last;
         }
+        const uint32_t launches = rs.launches;
         res->rounds = rounds;
         res->launches = launches;
         res->device_seconds = device_seconds;

@@ -122,6 +122,24 @@ CostEstimate HipDenseEngine::estimate(const ExecutionRequest& r) const noexcept 
     return e;
 }
 
+std::map<uint64_t, uint64_t> sample_counts(const void* state, uint32_t n, int32_t precision, uint64_t shots,
+                                           uint64_t seed, const std::vector<std::pair<uint32_t, uint32_t>>& targets) {
+    std::mt19937_64 rng(seed);
+    std::vector<double> u(shots);
+    for (auto& x : u) x = word_to_uniform(rng());
+    std::sort(u.begin(), u.end());
+    std::vector<uint64_t> basis(shots);
+    sample_sorted(state, n, precision, u.data(), shots, basis.data());
+    std::map<uint64_t, uint64_t> counts;
+    for (uint64_t b : basis) {
+        uint64_t key = 0;
+        for (const auto& [q, c] : targets)
+            if ((b >> q) & 1ULL) key |= uint64_t{1} << c;
+        counts[key] += 1;
+    }
+    return counts;
+}
+
 ExecutionResult HipDenseEngine::execute(const ExecutionRequest& r) const {
     using clock = std::chrono::steady_clock;
     const auto t0 = clock::now();
@@ -169,20 +187,7 @@ ExecutionResult HipDenseEngine::execute(const ExecutionRequest& r) const {
 
         if (r.shots > 0) {
             const auto s0 = clock::now();
-            std::mt19937_64 rng(r.seed);
-            std::vector<double> u(r.shots);
-            for (auto& x : u) x = word_to_uniform(rng());
-            std::sort(u.begin(), u.end());
-            std::vector<uint64_t> basis(r.shots);
-            sample_sorted(state.get(), n, r.precision, u.data(), r.shots, basis.data());
-            std::map<uint64_t, uint64_t> counts;
-            for (uint64_t b : basis) {
-                uint64_t key = 0;
-                for (const auto& [q, c] : r.measure_targets)
-                    if ((b >> q) & 1ULL) key |= uint64_t{1} << c;
-                counts[key] += 1;
-            }
-            result.counts = std::move(counts);
+            result.counts = sample_counts(state.get(), n, r.precision, r.shots, r.seed, r.measure_targets);
             result.sample_seconds = std::chrono::duration<double>(clock::now() - s0).count();
         }
         if (!r.terms.empty()) {

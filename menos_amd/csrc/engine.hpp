@@ -90,6 +90,13 @@ struct CostEstimate {
     double seconds = 0;
 };
 
+// Draws `shots` outcomes from a device state and folds them into counts keyed
+// by the clbit string (reference src/engines/dense/sample.cpp:24-75):
+// mt19937_64(seed) words as uniforms, sorted, then the device cumulative walk;
+// outcome bit q lands on clbit c for every (q, c) target.
+std::map<uint64_t, uint64_t> sample_counts(const void* state, uint32_t n, int32_t precision, uint64_t shots,
+                                           uint64_t seed, const std::vector<std::pair<uint32_t, uint32_t>>& targets);
+
 class HipDenseEngine {
 public:
     const char* name() const { return "dense_hip"; }

@@ -26,8 +26,11 @@ EXPORTS = (
     "mh_pool_reusable_bytes", "mh_pool_trim", "mh_state_init", "mh_apply_matrix", "mh_apply_matrix_generic",
     "mh_apply_blocks", "mh_expectation", "mh_sample", "mh_sample_unsorted", "mh_copy_to_host",
     "mh_copy_from_host", "mh_state_digest", "mh_engine_estimate", "mh_engine_execute", "mh_plan_fusion",
-    "mh_gate_matrix", "mh_calibrate", "mh_batch_execute", "mh_state_max_abs_diff",
+    "mh_gate_matrix", "mh_calibrate", "mh_batch_execute", "mh_state_max_abs_diff", "mh_server_create",
+    "mh_server_submit", "mh_server_wait", "mh_server_replay", "mh_server_stats_get", "mh_server_destroy",
+    "mh_derive_seeds",
 )
+JOB_ESTIMATOR, JOB_SAMPLER = 0, 1
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -369,6 +372,154 @@ def batch_execute(circuits, terms, *, precision=C128, fusion=FUSE_CONTAINMENT, f
             r.statevectors.append(sv[at:at + s].copy())
             at += s
     return r
+
+
+class CServerOptions(ctypes.Structure):
+    _fields_ = [("budget_bytes", ctypes.c_uint64), ("max_round_jobs", ctypes.c_uint32),
+                ("host_threads", ctypes.c_uint32), ("fusion", ctypes.c_int32), ("fusion_width", ctypes.c_uint32),
+                ("plan_cache_capacity", ctypes.c_uint32)]
+
+
+class CJob(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("precision", ctypes.c_int32), ("circuit", ctypes.POINTER(CCircuit)),
+                ("global_phase", ctypes.c_double), ("num_observables", ctypes.c_uint32),
+                ("observable_offsets", _u32p), ("x_masks", _u64p), ("z_masks", _u64p), ("coefficients", _f64p),
+                ("shots", ctypes.c_uint64), ("seed", ctypes.c_uint64), ("num_targets", ctypes.c_uint32),
+                ("target_qubits", _u32p), ("target_clbits", _u32p), ("bitstring_width", ctypes.c_uint32)]
+
+
+class CJobResult(ctypes.Structure):
+    _fields_ = [("observable_values", _f64p), ("raw_terms", _f64p), ("outcomes", _u64p), ("counts", _u64p),
+                ("status", ctypes.c_int32), ("error", ctypes.c_char * 256), ("num_outcomes", ctypes.c_uint64),
+                ("round", ctypes.c_uint32), ("plan_cache_hit", ctypes.c_int32),
+                ("submitted_seconds", ctypes.c_double), ("started_seconds", ctypes.c_double),
+                ("completed_seconds", ctypes.c_double)]
+
+
+class CServerStats(ctypes.Structure):
+    _fields_ = [("jobs_completed", ctypes.c_uint64), ("jobs_failed", ctypes.c_uint64), ("rounds", ctypes.c_uint32),
+                ("plan_cache_hits", ctypes.c_uint64), ("plan_cache_misses", ctypes.c_uint64),
+                ("peak_round_bytes", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64),
+                ("busy_seconds", ctypes.c_double)]
+
+
+def derive_seeds(seed: int, count: int) -> list[int]:
+    """The first `count` SplitMix64 outputs of `seed` (mh_derive_seeds)."""
+    out = np.zeros(max(count, 1), np.uint64)
+    check(lib().mh_derive_seeds(ctypes.c_uint64(seed), ctypes.c_uint32(count), _ptr(out, _u64p)))
+    return [int(v) for v in out[:count]]
+
+
+class Job:
+    """A served job and the host arrays its C form points into.
+
+    Estimator: ``observables`` is a list of observables, each a list of
+    ``(x_mask, z_mask, coefficient)`` terms. Sampler: ``shots`` outcomes drawn
+    with ``seed``; ``targets`` are (qubit, clbit) pairs (the circuit's final
+    measurements when omitted) and ``width`` the bit-string width."""
+
+    def __init__(self, kind, circuit, *, precision=C128, observables=(), shots=0, seed=0, targets=None,
+                 width=None):
+        self.kind, self.circuit, self.precision = kind, circuit, precision
+        self._keep = _CircuitKeep(circuit)
+        self.num_observables = len(observables)
+        offs = [0]
+        xs, zs, cs = [], [], []
+        for obs in observables:
+            for x, z, c in obs:
+                xs.append(x)
+                zs.append(z)
+                cs.extend((complex(c).real, complex(c).imag))
+            offs.append(len(xs))
+        self.num_terms = len(xs)
+        self._offs = np.array(offs, np.uint32)
+        self._x = np.array(xs or [0], np.uint64)
+        self._z = np.array(zs or [0], np.uint64)
+        self._c = np.array(cs or [0.0, 0.0], np.float64)
+        if targets is None:
+            targets = circuit.measure_targets()
+        if width is None:
+            width = max([c for _, c in targets], default=-1) + 1
+        self.shots = shots
+        self._tq = np.array([q for q, _ in targets] or [0], np.uint32)
+        self._tc = np.array([c for _, c in targets] or [0], np.uint32)
+        self.c = CJob(kind, precision, ctypes.pointer(self._keep.c), circuit.global_phase, self.num_observables,
+                      _ptr(self._offs, _u32p), _ptr(self._x, _u64p), _ptr(self._z, _u64p), _ptr(self._c, _f64p),
+                      shots, seed, len(targets), _ptr(self._tq, _u32p), _ptr(self._tc, _u32p), width)
+
+    def result_buffers(self):
+        vals = np.zeros(2 * max(self.num_observables, 1), np.float64)
+        raw = np.zeros(2 * max(self.num_terms, 1), np.float64)
+        outc = np.zeros(max(self.shots, 1), np.uint64)
+        cnts = np.zeros(max(self.shots, 1), np.uint64)
+        r = CJobResult(_ptr(vals, _f64p), _ptr(raw, _f64p), _ptr(outc, _u64p), _ptr(cnts, _u64p))
+        return r, (vals, raw, outc, cnts)
+
+    def collect(self, r: CJobResult, bufs) -> Result:
+        vals, raw, outc, cnts = bufs
+        out = Result(status=r.status, error=r.error.decode(), round=r.round, plan_cache_hit=bool(r.plan_cache_hit),
+                     submitted=r.submitted_seconds, started=r.started_seconds, completed=r.completed_seconds,
+                     latency=r.completed_seconds - r.submitted_seconds)
+        if r.status == MH_OK:
+            out.values = vals[:2 * self.num_observables].view(np.complex128).copy()
+            out.raw_terms = raw[:2 * self.num_terms].view(np.complex128).copy()
+            m = int(r.num_outcomes)
+            out.counts = {int(k): int(v) for k, v in zip(outc[:m], cnts[:m])} if self.kind == JOB_SAMPLER else None
+        return out
+
+
+class Server:
+    """A resident job server (mh_server_*): submit jobs, wait for results."""
+
+    def __init__(self, *, budget_bytes=0, max_round_jobs=0, host_threads=0, fusion=FUSE_CONTAINMENT,
+                 fusion_width=0, plan_cache_capacity=0):
+        o = CServerOptions(budget_bytes, max_round_jobs, host_threads, fusion, fusion_width, plan_cache_capacity)
+        self._h = ctypes.c_void_p()
+        check(lib().mh_server_create(ctypes.byref(o), ctypes.byref(self._h)))
+        self._pending = {}
+
+    def submit(self, job: Job) -> int:
+        t = ctypes.c_uint64()
+        check(lib().mh_server_submit(self._h, ctypes.byref(job.c), ctypes.byref(t)))
+        self._pending[t.value] = job
+        return t.value
+
+    def wait(self, ticket: int) -> Result:
+        job = self._pending.pop(ticket)
+        r, bufs = job.result_buffers()
+        check(lib().mh_server_wait(self._h, ctypes.c_uint64(ticket), ctypes.byref(r)))
+        return job.collect(r, bufs)
+
+    def replay(self, jobs, arrivals) -> list:
+        """Submits jobs[i] at arrivals[i] seconds (from the call's start) and
+        waits for all of them; returns their results in order."""
+        n = len(jobs)
+        arr = (CJob * max(n, 1))(*[j.c for j in jobs])
+        res = (CJobResult * max(n, 1))()
+        bufs = []
+        for i, j in enumerate(jobs):
+            r, b = j.result_buffers()
+            res[i] = r
+            bufs.append(b)
+        t = np.ascontiguousarray(arrivals, np.float64)
+        check(lib().mh_server_replay(self._h, ctypes.c_uint32(n), arr, _ptr(t if n else np.zeros(1), _f64p), res))
+        return [j.collect(res[i], bufs[i]) for i, j in enumerate(jobs)]
+
+    def stats(self) -> CServerStats:
+        s = CServerStats()
+        check(lib().mh_server_stats_get(self._h, ctypes.byref(s)))
+        return s
+
+    def close(self):
+        if self._h:
+            check(lib().mh_server_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def calibrate() -> Calibration:

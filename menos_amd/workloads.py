@@ -306,3 +306,115 @@ def batch_workload(count: int, seed: int, qubits=(10, 22), depth=(4, 12), terms:
         circuits.append(FlatAnsatz(n, d, angles))
         paulis.append(random_paulis(n, terms, seed + i, max_weight=max_weight, min_weight=1))
     return circuits, paulis
+
+
+class FlatBrick:
+    """``random_circuit``'s shape (U3 on every qubit, then a CZ brick, per
+    layer) built directly in the flat layout, with the angles given
+    (``3 * n * layers`` values in (layer, qubit, theta -> phi -> lambda)
+    order), then a measurement of qubit q into clbit q for every qubit."""
+
+    def __init__(self, num_qubits: int, layers: int, angles: np.ndarray):
+        n, d = int(num_qubits), int(layers)
+        self.num_qubits = n
+        self.num_clbits = n
+        self.global_phase = 0.0
+        ops, qs, arity, npar = [], [], [], []
+        for layer in range(d):
+            ops.append(np.full(n, U))
+            qs.append(np.arange(n))
+            arity.append(np.ones(n, np.uint32))
+            npar.append(np.full(n, 3, np.uint32))
+            lo = np.arange(layer % 2, n - 1, 2)
+            ops.append(np.full(lo.size, CZ))
+            qs.append(np.stack([lo, lo + 1], axis=1).reshape(-1))
+            arity.append(np.full(lo.size, 2, np.uint32))
+            npar.append(np.zeros(lo.size, np.uint32))
+        ops.append(np.full(n, MEASURE))
+        qs.append(np.stack([np.arange(n), np.arange(n)], axis=1).reshape(-1))
+        arity.append(np.full(n, 2, np.uint32))
+        npar.append(np.zeros(n, np.uint32))
+        self._opcodes = np.concatenate(ops).astype(np.int32)
+        self._qubits = np.concatenate(qs).astype(np.uint32)
+        self._qo = np.concatenate([[0], np.cumsum(np.concatenate(arity))]).astype(np.uint32)
+        self._po = np.concatenate([[0], np.cumsum(np.concatenate(npar))]).astype(np.uint32)
+        self._params = np.ascontiguousarray(angles, np.float64)
+        assert self._params.size == 3 * n * d
+        self.ops = range(self._opcodes.size)  # len() only
+
+    def flat(self):
+        return self._opcodes, self._qo, self._qubits, self._po, self._params
+
+    def measure_targets(self):
+        return [(q, q) for q in range(self.num_qubits)]
+
+
+@dataclass
+class ServeJobSpec:
+    kind: str                 # "estimator" or "sampler"
+    circuit: object           # FlatAnsatz or FlatBrick
+    observables: list         # estimator: [[(x, z, coefficient), ...], ...]
+    shots: int
+    seed: int
+    arrival: float            # seconds from the start
+    structure: int            # index of the structure it shares (parameter-only update when reused)
+    update: bool              # a parameter-only update of an earlier structure
+
+
+def serve_workload(count: int, rate: float, seed: int, *, qubits=(12, 30), estimator_share=0.7,
+                   update_share=0.4, observables=16, max_terms=4, max_weight=4, ansatz_depth=(4, 12),
+                   sampler_layers=20, shots=1000, job_seed=2026):
+    """The served workload: ``count`` jobs arriving as a Poisson process of
+    ``rate`` jobs/s (exponential gaps, numpy PCG64 from ``seed``).
+
+    * n is log-uniform on ``qubits``: ``floor(exp(U[ln lo, ln(hi + 1))))``;
+    * ``estimator_share`` of new structures are estimator jobs: an ansatz
+      (RY, RZ on every qubit, CNOT ladder; depth uniform on ``ansatz_depth``)
+      with ``observables`` observables of 1..``max_terms`` Pauli terms (weight
+      1..``max_weight``, coefficients uniform on [-1, 1]); the rest are sampler
+      jobs: ``sampler_layers`` layers of U3 + CZ brick, measure all, ``shots``
+      shots, seeds the SplitMix64 stream of ``job_seed`` (a sampler's pub seeds);
+    * with probability ``update_share`` a job (after the first) is a
+      parameter-only update: an earlier job's structure with fresh angles (and,
+      for an estimator, the same observables).
+    """
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, count)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    lo, hi = qubits
+    seeds = splitmix64_stream(job_seed, count)
+    structures = []  # (kind, n, depth, observables)
+    jobs = []
+    for i in range(count):
+        if structures and rng.random() < update_share:
+            s = int(rng.integers(0, len(structures)))
+            kind, n, depth, obs = structures[s]
+            update = True
+        else:
+            n = int(math.floor(math.exp(rng.uniform(math.log(lo), math.log(hi + 1)))))
+            n = min(max(n, lo), hi)
+            if rng.random() < estimator_share:
+                kind = "estimator"
+                depth = int(rng.integers(ansatz_depth[0], ansatz_depth[1] + 1))
+                obs = []
+                for _ in range(observables):
+                    terms = []
+                    for _ in range(int(rng.integers(1, max_terms + 1))):
+                        w = int(rng.integers(1, min(max_weight, n) + 1))
+                        qs = rng.choice(n, w, replace=False)
+                        spec = {int(q): "XYZ"[int(rng.integers(0, 3))] for q in qs}
+                        x, z = pauli_masks(spec)
+                        terms.append((x, z, float(rng.uniform(-1.0, 1.0))))
+                    obs.append(terms)
+            else:
+                kind, depth, obs = "sampler", sampler_layers, []
+            structures.append((kind, n, depth, obs))
+            s = len(structures) - 1
+            update = False
+        if kind == "estimator":
+            c = FlatAnsatz(n, depth, rng.uniform(-math.pi, math.pi, 2 * n * depth))
+            jobs.append(ServeJobSpec(kind, c, obs, 0, 0, float(arrivals[i]), s, update))
+        else:
+            c = FlatBrick(n, depth, rng.uniform(0.0, 2.0 * math.pi, 3 * n * depth))
+            jobs.append(ServeJobSpec(kind, c, [], shots, seeds[i], float(arrivals[i]), s, update))
+    return jobs

@@ -327,3 +327,35 @@ def counts_from_indices(indices, targets, width=None) -> dict[int, int]:
                 key |= 1 << c
         counts[key] = counts.get(key, 0) + 1
     return counts
+
+
+def combine_expectation(raw_terms, coefficients) -> complex:
+    """One observable's value from its raw, coefficient-free terms (reference
+    src/runtime/combine_expectation.cpp:7-25, one region): total starts at
+    (0, 0); per term in order, product = (1, 0) * term and
+    total += coefficient * product, each complex product the naive
+    (a*c - b*d, a*d + b*c) of std::complex on finite values. Returns
+    (re, im) as Python floats (IEEE doubles, the same operations)."""
+    tr, ti = 0.0, 0.0
+    for term, coef in zip(raw_terms, coefficients):
+        er, ei = float(complex(term).real), float(complex(term).imag)
+        pr = 1.0 * er - 0.0 * ei
+        pi = 1.0 * ei + 0.0 * er
+        cr, ci = float(complex(coef).real), float(complex(coef).imag)
+        tr = tr + (cr * pr - ci * pi)
+        ti = ti + (cr * pi + ci * pr)
+    return tr, ti
+
+
+def splitmix64(seed: int, count: int) -> list[int]:
+    """The first `count` outputs of SplitMix64 from `seed` (reference
+    src/common/splitmix64.h:15-28; Vigna's published splitmix64.c)."""
+    mask = (1 << 64) - 1
+    state, out = seed & mask, []
+    for _ in range(count):
+        state = (state + 0x9E3779B97F4A7C15) & mask
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+        out.append(z ^ (z >> 31))
+    return out
