@@ -137,7 +137,7 @@ def served(args, H, W) -> dict:
            "jobs": len(jobs),
            "estimators": sum(s.kind == "estimator" for s in specs),
            "updates": sum(s.update for s in specs)}
-    with H.Server(budget_bytes=256_000_000_000) as srv:
+    with H.Server(budget_bytes=256_000_000_000, reserve_bytes=256_000_000_000) as srv:
         w0 = time.perf_counter()
         res = srv.replay(jobs, np.zeros(len(jobs)))
         wall = time.perf_counter() - w0
@@ -148,12 +148,17 @@ def served(args, H, W) -> dict:
     cap = len(jobs) / wall
     out["capacity"] = {"circuits_per_s": round(cap, 2), "seconds": round(wall, 3), "rounds": st.rounds,
                        "plan_cache_hits": st.plan_cache_hits, "plan_cache_misses": st.plan_cache_misses,
-                       "peak_round_GB": round(st.peak_round_bytes / 1e9, 2)}
+                       "peak_round_GB": round(st.peak_round_bytes / 1e9, 2),
+                       "phase_seconds": {"plan": round(st.plan_seconds, 3), "arena": round(st.arena_seconds, 3),
+                                         "compile": round(st.compile_seconds, 3),
+                                         "passes": round(st.pass_seconds, 3),
+                                         "expectation": round(st.expectation_seconds, 3),
+                                         "sample": round(st.sample_seconds, 3)}}
     log(f"served capacity: {out['capacity']}")
     rate = args.serve_rate or 0.8 * cap
     arrivals = gaps / rate
     jobs = jobs_of()
-    with H.Server(budget_bytes=256_000_000_000) as srv:
+    with H.Server(budget_bytes=256_000_000_000, reserve_bytes=256_000_000_000) as srv:
         w0 = time.perf_counter()
         res = srv.replay(jobs, arrivals)
         wall = time.perf_counter() - w0

@@ -277,6 +277,7 @@ typedef struct mh_batch_request {
     uint64_t round_bytes;          /* device bytes of one round's states; 0: half the available memory */
     uint32_t host_threads;         /* planning threads; 0: the hardware's, at most 16 */
     int32_t want_statevectors;     /* copy every final state into result->statevectors */
+    uint32_t max_pass_cost;        /* FP operations per amplitude one HBM pass may carry; 0 = default */
 } mh_batch_request;
 
 typedef struct mh_batch_result {
@@ -286,7 +287,9 @@ typedef struct mh_batch_result {
     uint32_t rounds;
     uint32_t launches;
     double plan_seconds;           /* host: fusion plans and block matrices */
-    double device_seconds;         /* device: init, passes and reductions of every round */
+    double device_seconds;         /* device: init, passes and reductions of every round,
+                                      with the host's pass programs (compile_seconds) */
+    double compile_seconds;        /* host: window-pass programs and launch tables */
     double total_seconds;
 } mh_batch_result;
 
@@ -320,6 +323,8 @@ typedef struct mh_server_options {
     int32_t fusion;                 /* mh_fusion */
     uint32_t fusion_width;
     uint32_t plan_cache_capacity;   /* cached fusion plans; 0: 256 */
+    uint64_t reserve_bytes;         /* device bytes for round states allocated at creation (at most the
+                                       budget); 0: the arena grows with the rounds */
 } mh_server_options;
 
 typedef enum mh_job_kind { MH_JOB_ESTIMATOR = 0, MH_JOB_SAMPLER = 1 } mh_job_kind;
@@ -369,7 +374,13 @@ typedef struct mh_server_stats {
     uint64_t plan_cache_misses;
     uint64_t peak_round_bytes;      /* the ledger's peak: the largest round's states */
     uint64_t budget_bytes;          /* the budget in force */
-    double busy_seconds;            /* wall time inside rounds */
+    double busy_seconds;            /* wall time inside rounds, split into: */
+    double plan_seconds;            /*   host: fusion plans (or cache hits) and block matrices */
+    double arena_seconds;           /*   growing the round-state arena */
+    double compile_seconds;         /*   host: window-pass programs */
+    double pass_seconds;            /*   device: initialisation and gate passes */
+    double expectation_seconds;     /*   device: term reductions */
+    double sample_seconds;          /*   device and host: sampler draws and counts */
 } mh_server_stats;
 
 int mh_server_create(const mh_server_options* options, mh_server** out);

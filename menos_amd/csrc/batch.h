@@ -15,6 +15,7 @@ namespace mh {
 struct RoundItem {
     uint32_t n = 0;
     const std::vector<Block>* blocks = nullptr;
+    uint32_t max_pass_cost = 0;  // 0: the default
     double global_phase = 0.0;
     // Expectation terms: results land at terms_out[2 * (term_out + t)].
     uint32_t num_terms = 0;
@@ -30,8 +31,14 @@ struct RoundItem {
     void* statevector = nullptr;
 };
 
+// Wall time of a round's phases (each ends with a synchronisation).
 struct RoundStats {
     uint32_t launches = 0;
+    double prepare_seconds = 0;      // host: the caller's prepare callbacks
+    double compile_seconds = 0;      // host: pass planning and tables
+    double pass_seconds = 0;         // initialisation and gate passes, compile excluded
+    double expectation_seconds = 0;
+    double sample_seconds = 0;
 };
 
 // Device bytes one state of a round takes in the round's arena.
@@ -43,8 +50,12 @@ uint64_t round_state_bytes(uint32_t n, int32_t precision);
 // Synchronises. Every item needs window_width(precision) to 30 qubits.
 // `arena` (device memory of at least the items' round_state_bytes in total)
 // holds the states; null takes an arena from the pool for the call.
-void execute_round(int32_t precision, const std::vector<RoundItem>& items, uint32_t threads, double* terms_out,
-                   RoundStats* stats, void* arena = nullptr);
+// The gate passes go to the device in groups of items: prepare(lo, hi), when
+// given, runs on the host just before items [lo, hi) are compiled (to fill in
+// their blocks), while the device runs the groups before them.
+void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t threads, double* terms_out,
+                   RoundStats* stats, void* arena = nullptr,
+                   const std::function<void(std::size_t, std::size_t)>& prepare = nullptr);
 
 // Runs body(i) for i in [0, count) on up to `threads` host threads; rethrows
 // the first exception.

@@ -79,8 +79,8 @@ uint64_t round_state_bytes(uint32_t n, int32_t precision) {
     return ((uint64_t{1} << n) * amp_bytes(precision) + kAlign - 1) / kAlign * kAlign;
 }
 
-void execute_round(int32_t precision, const std::vector<RoundItem>& items, uint32_t threads, double* terms_out,
-                   RoundStats* stats, void* arena_in) {
+void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t threads, double* terms_out,
+                   RoundStats* stats, void* arena_in, const std::function<void(std::size_t, std::size_t)>& prepare) {
     const std::size_t amp = amp_bytes(precision);
     const uint32_t width = window_width(precision);
     if (items.empty()) return;
@@ -105,7 +105,7 @@ void execute_round(int32_t precision, const std::vector<RoundItem>& items, uint3
         offsets[k] = at / amp;
         passes[k].state = arena + at;
         passes[k].n = it.n;
-        passes[k].blocks = it.blocks;
+        passes[k].max_cost = it.max_pass_cost;
         const std::complex<double> ph = host_phase(it.global_phase);
         phases[2 * k] = ph.real();
         phases[2 * k + 1] = ph.imag();
@@ -138,13 +138,38 @@ void execute_round(int32_t precision, const std::vector<RoundItem>& items, uint3
                            static_cast<const double*>(dphases.as<double>()), n_here);
     MH_HIP(hipGetLastError());
     // Apply, reduce, sample, copy back.
+    using clock = std::chrono::steady_clock;
+    const auto p0 = clock::now();
     BatchStats bs;
-    run_window_passes_batch(precision, passes, threads, &bs);
+    double prepare_seconds = 0;
+    {
+        // Groups of items: the host plans and compiles group g + 1 while the
+        // device runs group g.
+        BatchPassRunner runner(precision, threads);
+        const std::size_t groups = count >= 64 ? 4 : 1;
+        for (std::size_t g = 0; g < groups; ++g) {
+            const std::size_t lo = count * g / groups, hi = count * (g + 1) / groups;
+            if (prepare) {
+                const auto q0 = clock::now();
+                prepare(lo, hi);
+                prepare_seconds += std::chrono::duration<double>(clock::now() - q0).count();
+            }
+            for (std::size_t k = lo; k < hi; ++k) {
+                if (items[k].blocks == nullptr) fail(MH_ERR_INTERNAL, "a round item without blocks");
+                passes[k].blocks = items[k].blocks;
+            }
+            runner.add(passes.data() + lo, hi - lo, &bs);
+        }
+        runner.finish();
+    }
+    const auto p1 = clock::now();
     uint32_t launches = 2 + bs.launches;
     if (!exp_items.empty()) {
         expectation_batch(precision, exp_items, terms_out);
+        synchronize();
         launches += 2;
     }
+    const auto p2 = clock::now();
     for (std::size_t k = 0; k < count; ++k) {
         const RoundItem& it = items[k];
         if (it.shots > 0) {
@@ -156,7 +181,15 @@ void execute_round(int32_t precision, const std::vector<RoundItem>& items, uint3
         if (it.statevector) download(it.statevector, passes[k].state, (uint64_t{1} << it.n) * amp);
     }
     synchronize();
-    if (stats) stats->launches += launches;
+    if (stats) {
+        const auto p3 = clock::now();
+        stats->launches += launches;
+        stats->compile_seconds += bs.compile_seconds;
+        stats->prepare_seconds += prepare_seconds;
+        stats->pass_seconds += std::chrono::duration<double>(p1 - p0).count() - bs.compile_seconds - prepare_seconds;
+        stats->expectation_seconds += std::chrono::duration<double>(p2 - p1).count();
+        stats->sample_seconds += std::chrono::duration<double>(p3 - p2).count();
+    }
 }
 
 }  // namespace mh
@@ -181,20 +214,17 @@ extern "C" int mh_batch_execute(const mh_batch_request* req, mh_batch_result* re
         uint32_t threads = req->host_threads;
         if (threads == 0) threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
 
-        // 1. Plans.
-        std::vector<Planned> plans(count);
-        parallel_for(count, threads, [&](std::size_t i) {
+        // 1. Checks that need no plan; the plans are made round by round, a
+        // group of circuits at a time, while the device runs the group before.
+        for (uint32_t i = 0; i < count; ++i) {
             const mh_circuit* c = &req->circuits[i];
-            validate_circuit(c);
             if (c->num_qubits < width || c->num_qubits > 30)
                 fail(MH_ERR_INVALID, "a batched circuit needs " + std::to_string(width) + " to 30 qubits");
             if (req->term_offsets[i + 1] < req->term_offsets[i]) fail(MH_ERR_INVALID, "term_offsets must not decrease");
-            const CircuitView cv{c};
-            const FusionPlan fp = plan_fusion(cv, req->fusion, req->fusion_width);
-            plans[i].n = c->num_qubits;
-            plans[i].blocks = build_blocks(cv, fp);
-        });
-        res->plan_seconds = std::chrono::duration<double>(clock::now() - t0).count();
+        }
+        std::vector<Planned> plans(count);
+        for (uint32_t i = 0; i < count; ++i) plans[i].n = req->circuits[i].num_qubits;
+        double plan_seconds = 0;
 
         // 2. Rounds of states that fit round_bytes; 3. each round executed.
         uint64_t round_bytes = req->round_bytes;
@@ -221,7 +251,7 @@ extern "C" int mh_batch_execute(const mh_batch_request* req, mh_batch_result* re
             for (uint32_t i = first; i < last; ++i) {
                 RoundItem& it = items[i - first];
                 it.n = plans[i].n;
-                it.blocks = &plans[i].blocks;
+                it.max_pass_cost = req->max_pass_cost;
                 it.global_phase = req->global_phases ? req->global_phases[i] : 0.0;
                 it.num_terms = req->term_offsets[i + 1] - req->term_offsets[i];
                 it.x = req->x_masks + req->term_offsets[i];
@@ -232,7 +262,20 @@ extern "C" int mh_batch_execute(const mh_batch_request* req, mh_batch_result* re
                     sv_at += (uint64_t{1} << plans[i].n) * amp;
                 }
             }
-            execute_round(req->precision, items, threads, res->expectation_terms, &rs);
+            const uint32_t round_first = first;
+            auto plan_group = [&](std::size_t lo, std::size_t hi) {
+                const auto q0 = clock::now();
+                parallel_for(hi - lo, threads, [&](std::size_t k) {
+                    const uint32_t i = round_first + static_cast<uint32_t>(lo + k);
+                    const mh_circuit* c = &req->circuits[i];
+                    validate_circuit(c);
+                    const CircuitView cv{c};
+                    plans[i].blocks = build_blocks(cv, plan_fusion(cv, req->fusion, req->fusion_width));
+                    items[lo + k].blocks = &plans[i].blocks;
+                });
+                plan_seconds += std::chrono::duration<double>(clock::now() - q0).count();
+            };
+            execute_round(req->precision, items, threads, res->expectation_terms, &rs, nullptr, plan_group);
             device_seconds += std::chrono::duration<double>(clock::now() - r0).count();
             const double done = std::chrono::duration<double>(clock::now() - t0).count();
             if (res->completion_seconds)
@@ -243,7 +286,9 @@ extern "C" int mh_batch_execute(const mh_batch_request* req, mh_batch_result* re
         const uint32_t launches = rs.launches;
         res->rounds = rounds;
         res->launches = launches;
-        res->device_seconds = device_seconds;
+        res->plan_seconds = plan_seconds;
+        res->device_seconds = device_seconds - plan_seconds;
+        res->compile_seconds = rs.compile_seconds;
         res->total_seconds = std::chrono::duration<double>(clock::now() - t0).count();
     });
 }

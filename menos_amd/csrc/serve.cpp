@@ -206,6 +206,7 @@ public:
         MH_HIP(hipMemGetInfo(&free_b, &total_b));
         budget_ = std::min<uint64_t>(budget_, static_cast<uint64_t>(free_b) + pool_reusable_bytes());
         threads_ = o.host_threads ? o.host_threads : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        if (o.reserve_bytes) arena_ = DeviceBuffer(std::min(o.reserve_bytes, budget_));
         worker_ = std::thread([this] { loop(); });
     }
 
@@ -260,6 +261,12 @@ public:
         s->peak_round_bytes = peak_round_bytes_;
         s->budget_bytes = budget_;
         s->busy_seconds = busy_seconds_;
+        s->plan_seconds = plan_seconds_;
+        s->arena_seconds = arena_seconds_;
+        s->compile_seconds = phases_.compile_seconds;
+        s->pass_seconds = phases_.pass_seconds;
+        s->expectation_seconds = phases_.expectation_seconds;
+        s->sample_seconds = phases_.sample_seconds;
     }
 
 private:
@@ -434,8 +441,11 @@ private:
                 errors[i] = e.what();
             }
         });
+        const double planned = now();
         ensure_arena(bytes);
+        const double arena_ready = now();
         uint64_t arena_at = 0;
+        RoundStats rs;
         for (const int32_t precision : {MH_C128, MH_C64}) {
             std::vector<Job*> group;
             for (std::size_t i = 0; i < round.size(); ++i)
@@ -463,7 +473,6 @@ private:
             }
             std::vector<double> raw(2 * static_cast<std::size_t>(terms));
             try {
-                RoundStats rs;
                 execute_round(precision, items, threads_, raw.data(), &rs,
                               static_cast<unsigned char*>(arena_.get()) + arena_at);
             } catch (const Error& e) {
@@ -484,6 +493,12 @@ private:
             }
         }
         std::lock_guard<std::mutex> lock(mu_);
+        plan_seconds_ += planned - started;
+        arena_seconds_ += arena_ready - planned;
+        phases_.compile_seconds += rs.compile_seconds;
+        phases_.pass_seconds += rs.pass_seconds;
+        phases_.expectation_seconds += rs.expectation_seconds;
+        phases_.sample_seconds += rs.sample_seconds;
         for (std::size_t i = 0; i < round.size(); ++i)
             if (statuses[i] != MH_OK) finish(round[i], statuses[i], errors[i]);
     }
@@ -497,11 +512,13 @@ private:
             }
     }
 
-    // The server's arena grows to the largest round and is kept.
+    // The server's arena is kept between rounds; it grows at least fourfold
+    // (up to the budget) so that a rising load reallocates it a few times only.
     void ensure_arena(uint64_t bytes) {
         if (arena_.bytes() >= bytes) return;
+        const uint64_t grown = std::min(budget_, std::max(bytes, 4 * arena_.bytes()));
         arena_.reset();
-        arena_ = DeviceBuffer(bytes);
+        arena_ = DeviceBuffer(grown);
     }
 
     mh_server_options options_;
@@ -521,6 +538,9 @@ private:
     uint32_t rounds_ = 0;
     uint64_t peak_round_bytes_ = 0;
     double busy_seconds_ = 0;
+    double plan_seconds_ = 0;
+    double arena_seconds_ = 0;
+    RoundStats phases_;
     std::thread worker_;
 };
 

@@ -34,6 +34,7 @@
 // adding +0 to every component at the end of the last pass gives its bits
 // exactly. Complex64 passes use fused multiply-adds and carry a stated error.
 #include <algorithm>
+#include <chrono>
 #include <atomic>
 #include <cstdint>
 #include <cstring>
@@ -65,32 +66,41 @@ template <> struct Shape<float> {
 // A pass is a list of bundles; a bundle is a 64-bit set of stages that run in
 // the fixed order of their bit indices (B register bits):
 //   0 sign diagonal, 1 diagonal,
-//   for each register bit J, at 2 + 8J: lane bit I <-> J exchange (I = 0..5),
+//   for each register bit J, at 2 + 12J: lane bit I <-> J exchange (I = 0..5),
 //     then a 1-qubit block on J whose entry (0, 0) is real, then any 1-qubit
-//     block on J,
-//   2 + 8B: 2-qubit block on register bits 0, 1; 3 + 8B sign diagonal,
-//   4 + 8B diagonal; 5 + 8B + p: swap of register-bit pair p.
+//     block on J, then a controlled pair on J whose control is a lane bit or
+//     a qubit outside the window, then one whose control is register bit K
+//     (one slot per K != J),
+//   2 + 12B: 2-qubit block on register bits 0, 1; 3 + 12B sign diagonal,
+//   4 + 12B diagonal; 5 + 12B + p: swap of register-bit pair p.
+// A controlled pair is a 2-qubit block that never mixes the two values of one
+// operand (the control): a 2x2 block on the other operand, chosen by the
+// control's bit.
 // A loop over bundles whose body is this fixed chain of independent stages
 // keeps every amplitude in its registers from stage to stage; a switch over
 // op kinds instead makes the compiler move the whole window between
 // registers on every op.
 struct Stages {
-    static constexpr uint32_t kSignEarly = 0, kDiagEarly = 1;
-    static constexpr uint32_t group(uint32_t j) { return 2 + 8 * j; }
+    static constexpr uint32_t kSignEarly = 0, kDiagEarly = 1, kGroupSlots = 12;
+    static constexpr uint32_t group(uint32_t j) { return 2 + kGroupSlots * j; }
     static constexpr uint32_t lane_swap(uint32_t i, uint32_t j) { return group(j) + i; }
     static constexpr uint32_t dense1_re(uint32_t j) { return group(j) + 6; }
     static constexpr uint32_t dense1(uint32_t j) { return group(j) + 7; }
-    static constexpr uint32_t dense2(uint32_t rb) { return 2 + 8 * rb; }
-    static constexpr uint32_t sign_late(uint32_t rb) { return 3 + 8 * rb; }
-    static constexpr uint32_t diag_late(uint32_t rb) { return 4 + 8 * rb; }
+    static constexpr uint32_t pair_var(uint32_t j) { return group(j) + 8; }
+    static constexpr uint32_t pair_reg(uint32_t j, uint32_t k) { return group(j) + 9 + (k < j ? k : k - 1); }
+    static constexpr uint32_t dense2(uint32_t rb) { return 2 + kGroupSlots * rb; }
+    static constexpr uint32_t sign_late(uint32_t rb) { return 3 + kGroupSlots * rb; }
+    static constexpr uint32_t diag_late(uint32_t rb) { return 4 + kGroupSlots * rb; }
     static constexpr uint32_t pair_index(uint32_t j1, uint32_t j2, uint32_t rb) {
         uint32_t p = 0;
         for (uint32_t a = 0; a < j1; ++a) p += rb - 1 - a;
         return p + (j2 - j1 - 1);
     }
-    static constexpr uint32_t reg_swap(uint32_t j1, uint32_t j2, uint32_t rb) { return 5 + 8 * rb + pair_index(j1, j2, rb); }
+    static constexpr uint32_t reg_swap(uint32_t j1, uint32_t j2, uint32_t rb) { return 5 + kGroupSlots * rb + pair_index(j1, j2, rb); }
 };
-static_assert(Stages::reg_swap(3, 4, 5) < 64, "stages of one bundle fit 64 bits");
+static_assert(Stages::reg_swap(2, 3, 4) < 64, "stages of one bundle fit 64 bits");
+static_assert(Stages::pair_reg(0, 3) < Stages::group(1) && Stages::pair_reg(3, 2) < Stages::dense2(4),
+              "a register bit's stages fit its group");
 
 // The parameters of a stage that has them (blocks, not exchanges), in stage
 // order within a bundle. A sign stage applies a run of consecutive sign
@@ -98,10 +108,13 @@ static_assert(Stages::reg_swap(3, 4, 5) < 64, "stages of one bundle fit 64 bits"
 // one XOR mask over the lane's amplitudes.
 struct WOp {
     uint32_t mat;            // first entry in the matrix table; sign stage: first term
-    uint32_t num_outer;      // diagonal: operands outside the window; sign stage: terms
+    uint32_t num_outer;      // diagonal: operands outside the window; sign stage: terms;
+                             // controlled pair on a register control: 1 for a controlled NOT
     uint64_t neg_mask;       // sign diagonal: bit e set when entry e is -1
-    uint8_t lane_bit[8];     // diagonal: operand bit that lane bit i supplies, or kNone
-    uint8_t outer_q[8];      // diagonal: global qubit of the m-th operand outside the window
+    uint8_t lane_bit[8];     // diagonal: operand bit that lane bit i supplies, or kNone;
+                             // controlled pair: [0] the control's lane bit, or kNone
+    uint8_t outer_q[8];      // diagonal: global qubit of the m-th operand outside the window;
+                             // controlled pair: [0] the control when it is outside the window
     uint8_t outer_b[8];      // ... and its operand bit
     uint8_t rpart[32];       // diagonal: operand bits that register index r supplies
 };
@@ -258,30 +271,27 @@ __device__ __forceinline__ void reg_swap(V* v) {
     }
 }
 
-// One 1-qubit block on register bit J. Per pair every product is formed
-// before any sum, so the outputs are defined after the last read of the
-// inputs and take over their registers.
-template <uint32_t E, uint32_t J, bool kRe>
-__device__ __forceinline__ void dense1(double2* v, const double2* m) {
-    const double2 m00 = m[0], m01 = m[1], m10 = m[2], m11 = m[3];
-#pragma unroll
-    for (uint32_t r = 0; r < E; ++r) {
-        if (r & (1u << J)) continue;
-        const double2 a0 = v[r], a1 = v[r | (1u << J)];
-        const double p0 = m00.x * a0.x, p2 = m00.x * a0.y;
-        const double p1 = kRe ? 0.0 : m00.y * a0.y, p3 = kRe ? 0.0 : m00.y * a0.x;
-        const double p4 = m01.x * a1.x, p5 = m01.y * a1.y, p6 = m01.x * a1.y, p7 = m01.y * a1.x;
-        const double p8 = m10.x * a0.x, p9 = m10.y * a0.y, pa = m10.x * a0.y, pb = m10.y * a0.x;
-        const double pc = m11.x * a1.x, pd = m11.y * a1.y, pe = m11.x * a1.y, pf = m11.y * a1.x;
-        const double s0r = (kRe ? p0 : p0 - p1) + (p4 - p5);
-        const double s0i = (kRe ? p2 : p2 + p3) + (p6 + p7);
-        const double s1r = (p8 - p9) + (pc - pd);
-        const double s1i = (pa + pb) + (pe + pf);
-        v[r] = make_double2(s0r, s0i);
-        v[r | (1u << J)] = make_double2(s1r, s1i);
-    }
+// One pair (x0: the operand's bit clear, x1: set) under a 2x2 block. Every
+// product is formed before any sum, so the outputs are defined after the last
+// read of the inputs and take over their registers.
+template <bool kRe>
+__device__ __forceinline__ void pair1(double2& x0, double2& x1, double2 m00, double2 m01, double2 m10, double2 m11) {
+    const double2 a0 = x0, a1 = x1;
+    const double p0 = m00.x * a0.x, p2 = m00.x * a0.y;
+    const double p1 = kRe ? 0.0 : m00.y * a0.y, p3 = kRe ? 0.0 : m00.y * a0.x;
+    const double p4 = m01.x * a1.x, p5 = m01.y * a1.y, p6 = m01.x * a1.y, p7 = m01.y * a1.x;
+    const double p8 = m10.x * a0.x, p9 = m10.y * a0.y, pa = m10.x * a0.y, pb = m10.y * a0.x;
+    const double pc = m11.x * a1.x, pd = m11.y * a1.y, pe = m11.x * a1.y, pf = m11.y * a1.x;
+    const double s0r = (kRe ? p0 : p0 - p1) + (p4 - p5);
+    const double s0i = (kRe ? p2 : p2 + p3) + (p6 + p7);
+    const double s1r = (p8 - p9) + (pc - pd);
+    const double s1i = (pa + pb) + (pe + pf);
+    x0 = make_double2(s0r, s0i);
+    x1 = make_double2(s1r, s1i);
 }
 
+// (sr, si) += m * a in FP32 with fused multiply-adds (the complex64 form is
+// held to a tolerance, not to the reference's bits).
 __device__ __forceinline__ void cmac(float& sr, float& si, float2 m, float2 a) {
     sr = __builtin_fmaf(m.x, a.x, sr);
     sr = __builtin_fmaf(-m.y, a.y, sr);
@@ -289,24 +299,80 @@ __device__ __forceinline__ void cmac(float& sr, float& si, float2 m, float2 a) {
     si = __builtin_fmaf(m.y, a.x, si);
 }
 
-template <uint32_t E, uint32_t J, bool kRe>
-__device__ __forceinline__ void dense1(float2* v, const float2* m) {
-    const float2 m00 = m[0], m01 = m[1], m10 = m[2], m11 = m[3];
+template <bool kRe>
+__device__ __forceinline__ void pair1(float2& x0, float2& x1, float2 m00, float2 m01, float2 m10, float2 m11) {
+    const float2 a0 = x0, a1 = x1;
+    float s0r = m00.x * a0.x, s0i = m00.x * a0.y;
+    if constexpr (!kRe) {
+        s0r = __builtin_fmaf(-m00.y, a0.y, s0r);
+        s0i = __builtin_fmaf(m00.y, a0.x, s0i);
+    }
+    cmac(s0r, s0i, m01, a1);
+    float s1r = 0.0f, s1i = 0.0f;
+    cmac(s1r, s1i, m10, a0);
+    cmac(s1r, s1i, m11, a1);
+    x0 = make_float2(s0r, s0i);
+    x1 = make_float2(s1r, s1i);
+}
+
+// One 1-qubit block on register bit J.
+template <uint32_t E, uint32_t J, bool kRe, typename V>
+__device__ __forceinline__ void dense1(V* v, const V* m) {
+    const V m00 = m[0], m01 = m[1], m10 = m[2], m11 = m[3];
 #pragma unroll
     for (uint32_t r = 0; r < E; ++r) {
         if (r & (1u << J)) continue;
-        const float2 a0 = v[r], a1 = v[r | (1u << J)];
-        float s0r = m00.x * a0.x, s0i = m00.x * a0.y;
-        if constexpr (!kRe) {
-            s0r = __builtin_fmaf(-m00.y, a0.y, s0r);
-            s0i = __builtin_fmaf(m00.y, a0.x, s0i);
+        pair1<kRe>(v[r], v[r | (1u << J)], m00, m01, m10, m11);
+    }
+}
+
+// A controlled pair on register bit J whose control is register bit K:
+// m[0..3] is the block where the control is clear, m[4..7] where it is set.
+// A controlled NOT (identity, then the exchange) only exchanges amplitudes:
+// a product by an exact 1 or 0 and a sum with a signed zero change no value
+// (up to the sign of a zero).
+template <uint32_t E, uint32_t J, uint32_t K, typename V>
+__device__ __forceinline__ void pair_reg(V* v, const V* m, bool cnot) {
+    if (cnot) {
+#pragma unroll
+        for (uint32_t r = 0; r < E; ++r) {
+            if ((r & (1u << J)) || !((r >> K) & 1u)) continue;
+            const V t = v[r];
+            v[r] = v[r | (1u << J)];
+            v[r | (1u << J)] = t;
         }
-        cmac(s0r, s0i, m01, a1);
-        float s1r = 0.0f, s1i = 0.0f;
-        cmac(s1r, s1i, m10, a0);
-        cmac(s1r, s1i, m11, a1);
-        v[r] = make_float2(s0r, s0i);
-        v[r | (1u << J)] = make_float2(s1r, s1i);
+        return;
+    }
+    const V a00 = m[0], a01 = m[1], a10 = m[2], a11 = m[3];
+    const V b00 = m[4], b01 = m[5], b10 = m[6], b11 = m[7];
+#pragma unroll
+    for (uint32_t r = 0; r < E; ++r) {
+        if (r & (1u << J)) continue;
+        if ((r >> K) & 1u)
+            pair1<false>(v[r], v[r | (1u << J)], b00, b01, b10, b11);
+        else
+            pair1<false>(v[r], v[r | (1u << J)], a00, a01, a10, a11);
+    }
+}
+
+__device__ __forceinline__ double2 pick(bool set, double2 a, double2 b) {
+    return make_double2(set ? b.x : a.x, set ? b.y : a.y);
+}
+
+__device__ __forceinline__ float2 pick(bool set, float2 a, float2 b) {
+    return make_float2(set ? b.x : a.x, set ? b.y : a.y);
+}
+
+// A controlled pair on register bit J whose control bit `set` is the same for
+// all of the lane's amplitudes (a lane bit, or a qubit outside the window).
+template <uint32_t E, uint32_t J, typename V>
+__device__ __forceinline__ void pair_var(V* v, const V* m, bool set) {
+    const V m00 = pick(set, m[0], m[4]), m01 = pick(set, m[1], m[5]);
+    const V m10 = pick(set, m[2], m[6]), m11 = pick(set, m[3], m[7]);
+#pragma unroll
+    for (uint32_t r = 0; r < E; ++r) {
+        if (r & (1u << J)) continue;
+        pair1<false>(v[r], v[r | (1u << J)], m00, m01, m10, m11);
     }
 }
 
@@ -435,11 +501,22 @@ __device__ __forceinline__ void apply_diag(V* v, const WOp& op, const V* mats, u
     }
 }
 
-// The stages of register bit J (exchanges into J, then its 1-qubit blocks).
-template <typename V, uint32_t E, uint32_t J>
-__device__ __forceinline__ void group_stages(uint64_t c, V* v, const WOp*& op, const V* mats, uint32_t lane) {
-    if constexpr ((1u << J) < E) {
-        if (!(c & (0xFFull << Stages::group(J)))) return;
+template <typename V, uint32_t E, uint32_t RB, uint32_t J, uint32_t K>
+__device__ __forceinline__ void pair_reg_stage(uint64_t c, V* v, const WOp*& op, const V* mats) {
+    if constexpr (K != J && K < RB)
+        if (c & (1ull << Stages::pair_reg(J, K))) {
+            const WOp& o = *op++;
+            pair_reg<E, J, K>(v, mats + o.mat, o.num_outer != 0);
+        }
+}
+
+// The stages of register bit J (exchanges into J, then its 1-qubit blocks and
+// controlled pairs).
+template <typename V, uint32_t E, uint32_t RB, uint32_t J>
+__device__ __forceinline__ void group_stages(uint64_t c, V* v, const WOp*& op, const V* mats, uint64_t obase,
+                                             uint32_t lane) {
+    if constexpr (J < RB) {
+        if (!(c & (0xFFFull << Stages::group(J)))) return;
         if (c & (1ull << Stages::lane_swap(0, J))) lane_swap<V, E, 0, J>(v, lane);
         if (c & (1ull << Stages::lane_swap(1, J))) lane_swap<V, E, 1, J>(v, lane);
         if (c & (1ull << Stages::lane_swap(2, J))) lane_swap<V, E, 2, J>(v, lane);
@@ -448,6 +525,17 @@ __device__ __forceinline__ void group_stages(uint64_t c, V* v, const WOp*& op, c
         if (c & (1ull << Stages::lane_swap(5, J))) lane_swap<V, E, 5, J>(v, lane);
         if (c & (1ull << Stages::dense1_re(J))) dense1<E, J, true>(v, mats + (op++)->mat);
         if (c & (1ull << Stages::dense1(J))) dense1<E, J, false>(v, mats + (op++)->mat);
+        if (c & (1ull << Stages::pair_var(J))) {
+            const WOp& o = *op++;
+            const bool set = o.lane_bit[0] != kNone ? ((lane >> o.lane_bit[0]) & 1u) != 0
+                                                    : ((obase >> o.outer_q[0]) & 1ull) != 0;
+            pair_var<E, J>(v, mats + o.mat, set);
+        }
+        pair_reg_stage<V, E, RB, J, 0>(c, v, op, mats);
+        pair_reg_stage<V, E, RB, J, 1>(c, v, op, mats);
+        pair_reg_stage<V, E, RB, J, 2>(c, v, op, mats);
+        pair_reg_stage<V, E, RB, J, 3>(c, v, op, mats);
+        pair_reg_stage<V, E, RB, J, 4>(c, v, op, mats);
     }
 }
 
@@ -489,11 +577,11 @@ __device__ __forceinline__ void window_body(typename Shape<R>::V* state, const u
         const uint64_t c = bundles[b];
         if (c & (1ull << Stages::kSignEarly)) apply_signs<V, E>(v, *op++, terms, obase, lane);
         if (c & (1ull << Stages::kDiagEarly)) apply_diag<V, E>(v, *op++, mats, obase, lane);
-        group_stages<V, E, 0>(c, v, op, mats, lane);
-        group_stages<V, E, 1>(c, v, op, mats, lane);
-        group_stages<V, E, 2>(c, v, op, mats, lane);
-        group_stages<V, E, 3>(c, v, op, mats, lane);
-        group_stages<V, E, 4>(c, v, op, mats, lane);
+        group_stages<V, E, RB, 0>(c, v, op, mats, obase, lane);
+        group_stages<V, E, RB, 1>(c, v, op, mats, obase, lane);
+        group_stages<V, E, RB, 2>(c, v, op, mats, obase, lane);
+        group_stages<V, E, RB, 3>(c, v, op, mats, obase, lane);
+        group_stages<V, E, RB, 4>(c, v, op, mats, obase, lane);
         if (c & (1ull << Stages::dense2(RB))) dense2<E>(v, mats + (op++)->mat);
         if (c & (1ull << Stages::sign_late(RB))) apply_signs<V, E>(v, *op++, terms, obase, lane);
         if (c & (1ull << Stages::diag_late(RB))) apply_diag<V, E>(v, *op++, mats, obase, lane);
@@ -590,13 +678,67 @@ bool sign_diagonal(const Block& b) {
     return true;
 }
 
+// The control operand (0 or 1) of a 2-qubit block that never mixes that
+// operand's two values, i.e. every entry between rows and columns that differ
+// in its bit is zero, or -1. Such a block is a 2x2 block on the other operand
+// chosen by the control's bit (CX, CZ or a controlled rotation with 1-qubit
+// gates on its target fused in). The reference sums all four products of a
+// row in column order; the two it skips here are a zero entry times an
+// amplitude, a signed zero, and adding a signed zero leaves every other sum
+// unchanged up to the sign of a zero result, so the pair form is the same
+// arithmetic modulo zero signs, which the pass's final normalisation removes.
+int pair_control(const Block& b) {
+    if (b.diagonal || b.operands.size() != 2) return -1;
+    for (uint32_t u = 0; u < 2; ++u) {
+        bool ok = true;
+        for (uint32_t out = 0; out < 4 && ok; ++out)
+            for (uint32_t in = 0; in < 4 && ok; ++in)
+                if (((out ^ in) >> u) & 1u) ok = b.matrix[out * 4 + in] == cplx(0.0, 0.0);
+        if (ok) return static_cast<int>(u);
+    }
+    return -1;
+}
+
+// A controlled pair that is a controlled NOT: the identity where the control
+// is clear, the exchange where it is set (entries exactly 0 and 1).
+bool is_cnot(const std::vector<cplx>& m) {
+    const cplx one(1.0, 0.0), zero(0.0, 0.0);
+    return m[0] == one && m[1] == zero && m[2] == zero && m[3] == one && m[4] == zero && m[5] == one &&
+           m[6] == one && m[7] == zero;
+}
+
+// The 2x2 blocks of a controlled pair with control operand u: where the
+// control is clear (m[0..3]), then where it is set (m[4..7]), each in the
+// target's row / column order.
+std::vector<cplx> pair_blocks(const Block& b, uint32_t u) {
+    std::vector<cplx> m(8);
+    for (uint32_t cb = 0; cb < 2; ++cb)
+        for (uint32_t to = 0; to < 2; ++to)
+            for (uint32_t ti = 0; ti < 2; ++ti) {
+                const uint32_t out = (cb << u) | (to << (1 - u)), in = (cb << u) | (ti << (1 - u));
+                m[4 * cb + 2 * to + ti] = b.matrix[out * 4 + in];
+            }
+    return m;
+}
+
 // FP operations per amplitude a block costs in a pass (the pass cap unit),
 // with an allowance for the exchanges that bring its operands to registers.
 uint32_t block_cost(const Block& b) {
     const uint32_t k = static_cast<uint32_t>(b.operands.size());
     if (b.diagonal) return sign_diagonal(b) ? 2 : 6;
     if (k == 1) return (b.matrix[0].imag() == 0.0 ? 12 : 14) + 2;
+    if (const int u = pair_control(b); u >= 0) return is_cnot(pair_blocks(b, static_cast<uint32_t>(u))) ? 8 : 14 + 4;
     return 2 * (3 * (1u << k) - 1) + 4;
+}
+
+// The qubits a non-diagonal block needs in registers: its operands, or a
+// controlled pair's target alone.
+uint64_t register_mask(const Block& b) {
+    uint64_t m = 0;
+    const int u = pair_control(b);
+    for (uint32_t t = 0; t < b.operands.size(); ++t)
+        if (static_cast<int>(t) != u) m |= 1ull << b.operands[t];
+    return m;
 }
 
 uint64_t mask_of(const std::vector<uint32_t>& q) {
@@ -663,14 +805,18 @@ public:
                 emit_diag(b);
                 continue;
             }
-            if (k == 1) {
-                const uint8_t p = pos_of_[b.operands[0]];
+            const int u = pair_control(b);
+            if (k == 1 || u >= 0) {
+                const uint8_t p = pos_of_[b.operands[k == 1 ? 0 : 1 - u]];
                 int j = reg_index(p);
                 if (j < 0) {
                     j = victim(i, p);
                     lane_swap(static_cast<uint32_t>(lane_index(p)), static_cast<uint32_t>(j));
                 }
-                emit_dense1(b, static_cast<uint32_t>(j));
+                if (k == 1)
+                    emit_dense1(b, static_cast<uint32_t>(j));
+                else
+                    emit_pair(b, static_cast<uint32_t>(u), static_cast<uint32_t>(j));
             } else {
                 for (uint32_t t = 0; t < k; ++t) {
                     const uint8_t p = pos_of_[b.operands[t]];
@@ -711,8 +857,9 @@ private:
     bool uses(std::size_t i, uint8_t p) const {
         const Block& b = *blocks_[i];
         if (b.diagonal) return false;
-        for (uint32_t q : b.operands)
-            if (pos_of_[q] == p) return true;
+        const int u = pair_control(b);
+        for (uint32_t t = 0; t < b.operands.size(); ++t)
+            if (static_cast<int>(t) != u && pos_of_[b.operands[t]] == p) return true;
         return false;
     }
 
@@ -741,8 +888,10 @@ private:
         std::vector<uint8_t> regs;
         for (const Block* b : blocks_) {
             if (b->diagonal) continue;
-            for (uint32_t q : b->operands) {
-                const uint8_t p = pos_of_[q];
+            const int u = pair_control(*b);
+            for (uint32_t t = 0; t < b->operands.size(); ++t) {
+                if (static_cast<int>(t) == u) continue;
+                const uint8_t p = pos_of_[b->operands[t]];
                 if (p >= L && regs.size() < RB && std::find(regs.begin(), regs.end(), p) == regs.end())
                     regs.push_back(p);
             }
@@ -830,6 +979,29 @@ private:
         std::memset(&op, 0, sizeof(op));
         op.mat = push_matrix(b.matrix);
         put(b.matrix[0].imag() == 0.0 ? Stages::dense1_re(j) : Stages::dense1(j), -1, &op);
+    }
+
+    // A controlled pair on register bit j.
+    void emit_pair(const Block& b, uint32_t u, uint32_t j) {
+        const std::vector<cplx> m = pair_blocks(b, u);
+        WOp op;
+        std::memset(&op, 0, sizeof(op));
+        std::memset(op.lane_bit, kNone, sizeof(op.lane_bit));
+        op.mat = push_matrix(m);
+        op.num_outer = is_cnot(m) ? 1u : 0u;
+        const uint8_t p = pos_of_[b.operands[u]];
+        if (p == kNone) {
+            op.outer_q[0] = static_cast<uint8_t>(b.operands[u]);
+            put(Stages::pair_var(j), -1, &op);
+            return;
+        }
+        const int li = lane_index(p);
+        if (li >= 0) {
+            op.lane_bit[0] = static_cast<uint8_t>(li);
+            put(Stages::pair_var(j), -1, &op);
+            return;
+        }
+        put(Stages::pair_reg(j, static_cast<uint32_t>(reg_index(p))), -1, &op);
     }
 
     void emit_dense2(const Block& b) {
@@ -988,11 +1160,29 @@ void launch_window_passes(void* state, uint32_t n, const std::vector<Block>& blo
     }
 }
 
+// What a launched group of states needs until the stream has run it: its
+// device tables and the host copies they were uploaded from.
+struct BatchKeep {
+    std::vector<DeviceBuffer> device;
+    std::vector<std::shared_ptr<void>> host;
+    hipEvent_t copied = nullptr;
+};
+
+template <typename T>
+void keep_upload(BatchKeep& keep, DeviceBuffer& dst, std::vector<T>&& src, hipStream_t copy_stream) {
+    auto held = std::make_shared<std::vector<T>>(std::move(src));
+    MH_HIP(hipMemcpyAsync(dst.get(), held->data(), held->size() * sizeof(T), hipMemcpyHostToDevice, copy_stream));
+    keep.host.push_back(held);
+}
+
 // Many states, each with its own block list: pass p of every state runs in
 // one launch of window_batch_kernel (blocks the window kernel cannot apply run
-// on the generic kernel, state by state, after that launch).
+// on the generic kernel, state by state, after that launch). Returns without
+// waiting: the tables go up on the copy stream, the launches wait for them on
+// the compute stream, and `keep` holds what they read.
 template <typename R>
-void launch_window_passes_batch(const std::vector<BatchItem>& items, uint32_t threads, BatchStats* stats) {
+void launch_window_passes_batch(const BatchItem* items, std::size_t count, uint32_t threads, BatchStats* stats,
+                                BatchKeep& keep) {
     using V = typename Shape<R>::V;
     constexpr uint32_t W = width_of<R>();
     Context& ctx = context();
@@ -1009,7 +1199,7 @@ void launch_window_passes_batch(const std::vector<BatchItem>& items, uint32_t th
         std::vector<V> mats;
         std::vector<PassRec> passes;
     };
-    const std::size_t count = items.size();
+    const auto c0 = std::chrono::steady_clock::now();
     std::vector<Compiled> comp(count);
     std::atomic<std::size_t> next{0};
     std::exception_ptr error;
@@ -1020,7 +1210,8 @@ void launch_window_passes_batch(const std::vector<BatchItem>& items, uint32_t th
                 const BatchItem& it = items[i];
                 Compiled& c = comp[i];
                 const std::vector<WindowPass> passes =
-                    plan_window_passes(it.n, sizeof(R) == 8 ? MH_C128 : MH_C64, *it.blocks, W, kDefaultPassCost);
+                    plan_window_passes(it.n, sizeof(R) == 8 ? MH_C128 : MH_C64, *it.blocks, W,
+                                       it.max_cost ? it.max_cost : kBatchPassCost);
                 PassCompiler<R> compiler(c.bundles, c.ops, c.terms, c.mats);
                 for (std::size_t p = 0; p < passes.size(); ++p) {
                     PassRec rec;
@@ -1077,15 +1268,19 @@ void launch_window_passes_batch(const std::vector<BatchItem>& items, uint32_t th
     }
     DeviceBuffer dbundles(bundles.size() * sizeof(uint64_t)), dops(ops.size() * sizeof(WOp));
     DeviceBuffer dterms(terms.size() * sizeof(SignTerm)), dmats(mats.size() * sizeof(V));
-    upload(dbundles.get(), bundles.data(), bundles.size() * sizeof(uint64_t));
-    upload(dops.get(), ops.data(), ops.size() * sizeof(WOp));
-    upload(dterms.get(), terms.data(), terms.size() * sizeof(SignTerm));
-    upload(dmats.get(), mats.data(), mats.size() * sizeof(V));
+    const uint64_t* const dbundles_p = dbundles.as<uint64_t>();
+    const WOp* const dops_p = dops.as<WOp>();
+    const SignTerm* const dterms_p = dterms.as<SignTerm>();
+    const V* const dmats_p = dmats.as<V>();
+    keep_upload(keep, dbundles, std::move(bundles), ctx.copy_stream);
+    keep_upload(keep, dops, std::move(ops), ctx.copy_stream);
+    keep_upload(keep, dterms, std::move(terms), ctx.copy_stream);
+    keep_upload(keep, dmats, std::move(mats), ctx.copy_stream);
 
     // Launch lists: pass p of every state that has one. State offsets count
     // amplitudes from the lowest state address.
     uintptr_t base_addr = UINTPTR_MAX;
-    for (const BatchItem& it : items) base_addr = std::min(base_addr, reinterpret_cast<uintptr_t>(it.state));
+    for (std::size_t i = 0; i < count; ++i) base_addr = std::min(base_addr, reinterpret_cast<uintptr_t>(items[i].state));
     V* const state_base = reinterpret_cast<V*>(base_addr);
     std::vector<WBatchArgs> all_args;
     std::vector<uint32_t> all_groups;
@@ -1117,19 +1312,27 @@ void launch_window_passes_batch(const std::vector<BatchItem>& items, uint32_t th
     }
     DeviceBuffer dargs(std::max<std::size_t>(all_args.size(), 1) * sizeof(WBatchArgs));
     DeviceBuffer dgroups(std::max<std::size_t>(all_groups.size(), 1) * sizeof(uint32_t));
-    upload(dargs.get(), all_args.data(), all_args.size() * sizeof(WBatchArgs));
-    upload(dgroups.get(), all_groups.data(), all_groups.size() * sizeof(uint32_t));
+    const WBatchArgs* const dargs_p = dargs.as<WBatchArgs>();
+    const uint32_t* const dgroups_p = dgroups.as<uint32_t>();
+    keep_upload(keep, dargs, std::move(all_args), ctx.copy_stream);
+    keep_upload(keep, dgroups, std::move(all_groups), ctx.copy_stream);
+    MH_HIP(hipEventCreateWithFlags(&keep.copied, hipEventDisableTiming));
+    MH_HIP(hipEventRecord(keep.copied, ctx.copy_stream));
+    MH_HIP(hipStreamWaitEvent(ctx.stream, keep.copied, 0));
+    keep.device.push_back(std::move(dbundles));
+    keep.device.push_back(std::move(dops));
+    keep.device.push_back(std::move(dterms));
+    keep.device.push_back(std::move(dmats));
+    keep.device.push_back(std::move(dargs));
+    keep.device.push_back(std::move(dgroups));
 
+    if (stats) stats->compile_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
     uint32_t launches = 0;
     for (std::size_t p = 0; p < max_passes; ++p) {
         if (list_len[p] > 0) {
             hipLaunchKernelGGL(window_batch_kernel<R>, dim3(total_groups[p]), dim3(kWavesPerGroup * 64), 0, ctx.stream,
-                               static_cast<const WBatchArgs*>(dargs.as<WBatchArgs>() + args_at[p]),
-                               static_cast<const uint32_t*>(dgroups.as<uint32_t>() + groups_at[p]),
-                               static_cast<uint32_t>(list_len[p]), state_base,
-                               static_cast<const uint64_t*>(dbundles.as<uint64_t>()),
-                               static_cast<const WOp*>(dops.as<WOp>()), static_cast<const SignTerm*>(dterms.as<SignTerm>()),
-                               static_cast<const V*>(dmats.as<V>()));
+                               dargs_p + args_at[p], dgroups_p + groups_at[p], static_cast<uint32_t>(list_len[p]),
+                               state_base, dbundles_p, dops_p, dterms_p, dmats_p);
             MH_HIP(hipGetLastError());
             ++launches;
         }
@@ -1141,10 +1344,9 @@ void launch_window_passes_batch(const std::vector<BatchItem>& items, uint32_t th
             ++launches;
         }
     }
-    synchronize();
     if (stats) {
-        stats->launches = launches;
-        stats->max_passes = static_cast<uint32_t>(max_passes);
+        stats->launches += launches;
+        stats->max_passes = std::max(stats->max_passes, static_cast<uint32_t>(max_passes));
     }
 }
 
@@ -1185,7 +1387,7 @@ std::vector<WindowPass> plan_window_passes(uint32_t n, int32_t precision, const 
             continue;
         }
         const uint32_t c = block_cost(blk);
-        const uint64_t dense = blk.diagonal ? 0 : mask_of(blk.operands);
+        const uint64_t dense = blk.diagonal ? 0 : register_mask(blk);
         const uint64_t want = cur.dense_mask | dense;
         const bool fits = static_cast<uint32_t>(__builtin_popcountll(want | low_mask)) <= w;
         if (open && fits && cost + c <= max_cost) {
@@ -1208,16 +1410,54 @@ std::vector<WindowPass> plan_window_passes(uint32_t n, int32_t precision, const 
     return passes;
 }
 
+struct BatchPassRunner::Impl {
+    int32_t precision = MH_C128;
+    uint32_t threads = 1;
+    std::vector<BatchKeep> keeps;
+};
+
+BatchPassRunner::BatchPassRunner(int32_t precision, uint32_t threads) : impl_(std::make_unique<Impl>()) {
+    amp_bytes(precision);
+    impl_->precision = precision;
+    impl_->threads = threads;
+}
+
+BatchPassRunner::~BatchPassRunner() {
+    // Whatever was launched or copied reads the kept tables: wait before they go.
+    if (!impl_->keeps.empty()) {
+        (void)hipStreamSynchronize(context().copy_stream);
+        (void)hipStreamSynchronize(context().stream);
+    }
+    for (BatchKeep& k : impl_->keeps)
+        if (k.copied) (void)hipEventDestroy(k.copied);
+}
+
+void BatchPassRunner::add(const BatchItem* items, std::size_t count, BatchStats* stats) {
+    if (count == 0) return;
+    const uint32_t width = window_width(impl_->precision);
+    for (std::size_t i = 0; i < count; ++i)
+        if (items[i].n < width) fail(MH_ERR_INVALID, "a batched state has fewer qubits than one window");
+    impl_->keeps.emplace_back();
+    if (impl_->precision == MH_C128)
+        launch_window_passes_batch<double>(items, count, impl_->threads, stats, impl_->keeps.back());
+    else
+        launch_window_passes_batch<float>(items, count, impl_->threads, stats, impl_->keeps.back());
+}
+
+void BatchPassRunner::finish() {
+    MH_HIP(hipStreamSynchronize(context().copy_stream));
+    synchronize();
+    for (BatchKeep& k : impl_->keeps)
+        if (k.copied) (void)hipEventDestroy(k.copied);
+    impl_->keeps.clear();
+}
+
 void run_window_passes_batch(int32_t precision, const std::vector<BatchItem>& items, uint32_t threads,
                              BatchStats* stats) {
     if (items.empty()) return;
-    const uint32_t width = window_width(precision);
-    for (const BatchItem& it : items)
-        if (it.n < width) fail(MH_ERR_INVALID, "a batched state has fewer qubits than one window");
-    if (precision == MH_C128)
-        launch_window_passes_batch<double>(items, threads, stats);
-    else
-        launch_window_passes_batch<float>(items, threads, stats);
+    BatchPassRunner runner(precision, threads);
+    runner.add(items.data(), items.size(), stats);
+    runner.finish();
 }
 
 void run_window_passes(void* state, uint32_t n, int32_t precision, const std::vector<Block>& blocks, uint32_t w,

@@ -331,17 +331,18 @@ class CBatchRequest(ctypes.Structure):
                 ("precision", ctypes.c_int32), ("fusion", ctypes.c_int32), ("fusion_width", ctypes.c_uint32),
                 ("global_phases", _f64p), ("term_offsets", _u32p), ("x_masks", _u64p), ("z_masks", _u64p),
                 ("round_bytes", ctypes.c_uint64), ("host_threads", ctypes.c_uint32),
-                ("want_statevectors", ctypes.c_int32)]
+                ("want_statevectors", ctypes.c_int32), ("max_pass_cost", ctypes.c_uint32)]
 
 
 class CBatchResult(ctypes.Structure):
     _fields_ = [("expectation_terms", _f64p), ("statevectors", ctypes.c_void_p), ("completion_seconds", _f64p),
                 ("rounds", ctypes.c_uint32), ("launches", ctypes.c_uint32), ("plan_seconds", ctypes.c_double),
-                ("device_seconds", ctypes.c_double), ("total_seconds", ctypes.c_double)]
+                ("device_seconds", ctypes.c_double), ("compile_seconds", ctypes.c_double),
+                ("total_seconds", ctypes.c_double)]
 
 
 def batch_execute(circuits, terms, *, precision=C128, fusion=FUSE_CONTAINMENT, fusion_width=0, round_bytes=0,
-                  host_threads=0, statevectors=False):
+                  host_threads=0, statevectors=False, max_pass_cost=0):
     """Runs every circuit and reduces its Pauli terms (a list of (x, z) masks per
     circuit) through mh_batch_execute. Returns a Result with `expectation` (one
     complex array per circuit), `completion_seconds`, optionally `statevectors`
@@ -359,13 +360,15 @@ def batch_execute(circuits, terms, *, precision=C128, fusion=FUSE_CONTAINMENT, f
     sv = np.zeros(sum(sizes) if statevectors else 1, amp_dtype(precision))
     phases = np.array([c.global_phase for c in circuits] or [0.0], np.float64)
     req = CBatchRequest(len(circuits), arr, precision, fusion, fusion_width, _ptr(phases, _f64p), _ptr(offs, _u32p),
-                        _ptr(xs, _u64p), _ptr(zs, _u64p), round_bytes, host_threads, 1 if statevectors else 0)
+                        _ptr(xs, _u64p), _ptr(zs, _u64p), round_bytes, host_threads, 1 if statevectors else 0,
+                        max_pass_cost)
     res = CBatchResult(_ptr(out, _f64p), sv.ctypes.data_as(ctypes.c_void_p), _ptr(done, _f64p))
     check(lib().mh_batch_execute(ctypes.byref(req), ctypes.byref(res)))
     e = out[:2 * int(offs[-1])].view(np.complex128)
     r = Result(expectation=[e[offs[i]:offs[i + 1]].copy() for i in range(len(circuits))],
                completion_seconds=done[:len(circuits)].copy(), rounds=res.rounds, launches=res.launches,
-               plan_seconds=res.plan_seconds, device_seconds=res.device_seconds, total_seconds=res.total_seconds)
+               plan_seconds=res.plan_seconds, device_seconds=res.device_seconds, total_seconds=res.total_seconds,
+               compile_seconds=res.compile_seconds)
     if statevectors:
         at, r.statevectors = 0, []
         for s in sizes:
@@ -377,7 +380,7 @@ def batch_execute(circuits, terms, *, precision=C128, fusion=FUSE_CONTAINMENT, f
 class CServerOptions(ctypes.Structure):
     _fields_ = [("budget_bytes", ctypes.c_uint64), ("max_round_jobs", ctypes.c_uint32),
                 ("host_threads", ctypes.c_uint32), ("fusion", ctypes.c_int32), ("fusion_width", ctypes.c_uint32),
-                ("plan_cache_capacity", ctypes.c_uint32)]
+                ("plan_cache_capacity", ctypes.c_uint32), ("reserve_bytes", ctypes.c_uint64)]
 
 
 class CJob(ctypes.Structure):
@@ -400,7 +403,9 @@ class CServerStats(ctypes.Structure):
     _fields_ = [("jobs_completed", ctypes.c_uint64), ("jobs_failed", ctypes.c_uint64), ("rounds", ctypes.c_uint32),
                 ("plan_cache_hits", ctypes.c_uint64), ("plan_cache_misses", ctypes.c_uint64),
                 ("peak_round_bytes", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64),
-                ("busy_seconds", ctypes.c_double)]
+                ("busy_seconds", ctypes.c_double), ("plan_seconds", ctypes.c_double), ("arena_seconds", ctypes.c_double),
+                ("compile_seconds", ctypes.c_double), ("pass_seconds", ctypes.c_double),
+                ("expectation_seconds", ctypes.c_double), ("sample_seconds", ctypes.c_double)]
 
 
 def derive_seeds(seed: int, count: int) -> list[int]:
@@ -472,8 +477,9 @@ class Server:
     """A resident job server (mh_server_*): submit jobs, wait for results."""
 
     def __init__(self, *, budget_bytes=0, max_round_jobs=0, host_threads=0, fusion=FUSE_CONTAINMENT,
-                 fusion_width=0, plan_cache_capacity=0):
-        o = CServerOptions(budget_bytes, max_round_jobs, host_threads, fusion, fusion_width, plan_cache_capacity)
+                 fusion_width=0, plan_cache_capacity=0, reserve_bytes=0):
+        o = CServerOptions(budget_bytes, max_round_jobs, host_threads, fusion, fusion_width, plan_cache_capacity,
+                           reserve_bytes)
         self._h = ctypes.c_void_p()
         check(lib().mh_server_create(ctypes.byref(o), ctypes.byref(self._h)))
         self._pending = {}

@@ -127,6 +127,51 @@ def test_runs_of_sign_diagonals(orc, hip):
             assert got.tobytes() == want.tobytes(), (n, trial)
 
 
+def _controlled(rng, u, cnot):
+    """A 2-qubit block that never mixes operand u's values: a random 2x2 (or
+    the identity then X, a controlled NOT) on the other operand per value of
+    operand u, with signed-zero entries elsewhere."""
+    m = np.zeros((4, 4), np.complex128)
+    for cb in range(2):
+        blk = (np.eye(2) if cb == 0 else np.array([[0, 1], [1, 0]])) if cnot else random_unitary(1, rng)
+        for to in range(2):
+            for ti in range(2):
+                m[(cb << u) | (to << (1 - u)), (cb << u) | (ti << (1 - u))] = blk[to, ti]
+    if not cnot:
+        m[m == 0] = -0.0
+    return m
+
+
+def test_controlled_pairs(orc, hip):
+    """Controlled pairs (2-qubit blocks that are a 2x2 block on one operand
+    chosen by the other's bit, controlled NOTs among them) whose control is a
+    register bit, a lane bit or a qubit outside the window, between dense
+    1- and 2-qubit blocks: bit for bit the generic kernel's result."""
+    rng = np.random.default_rng(23)
+    for n in (10, 12, 14):
+        for trial in range(3):
+            blocks = []
+            for _ in range(50):
+                r = rng.random()
+                ops = rng.permutation(n)[:2].tolist()
+                if r < 0.5:
+                    blocks.append((_controlled(rng, int(rng.integers(0, 2)), rng.random() < 0.4), ops))
+                elif r < 0.8:
+                    blocks.append((random_unitary(1, rng), ops[:1]))
+                else:
+                    blocks.append((random_unitary(2, rng), ops))
+            s = random_state(n, 40 + trial)
+            d = hip.DeviceState(n)
+            d.upload(s)
+            d.apply_blocks(blocks)
+            got = d.download()
+            d.free()
+            want = s.copy()
+            for m, ops in blocks:
+                orc.apply_generic(want, m, ops)
+            assert got.tobytes() == want.tobytes(), (n, trial)
+
+
 def test_block_list_equals_gate_by_gate(orc, hip):
     """Many blocks per LDS-window pass give the bits of one launch per block."""
     rng = np.random.default_rng(99)
