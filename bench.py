@@ -71,11 +71,11 @@ def batched(args, H, W) -> dict:
     mh_batch_execute at complex64 and complex128. The error is the largest
     |<P>_c64 - <P>_c128| over every circuit and term. The per-circuit GPU path
     (one mh_engine_execute per circuit) is timed on the first
-    `args.per_circuit_sample` circuits."""
+    `args.per_circuit_sample` circuits. Each form runs once untimed first."""
     import numpy as np
     circuits, terms = W.batch_workload(args.batch_circuits, 2026)
-    warm = H.batch_execute(circuits[:64], terms[:64], precision=H.C64)
-    del warm
+    for prec in (H.C64, H.C128):  # one untimed run of the whole batch per precision (steady state)
+        H.batch_execute(circuits, terms, precision=prec)
     out = {"workload": f"{len(circuits)} ansatz circuits, n 10..22, depth 4..12, 8 Paulis each",
            "circuits": len(circuits)}
     res = {}
@@ -234,6 +234,8 @@ def main() -> int:
     ap.add_argument("--batch-circuits", type=int, default=4096)
     ap.add_argument("--per-circuit-sample", type=int, default=256)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--only", choices=["batched", "served", "complex64"],
+                    help="run one section alone and print its JSON object (for profiling)")
     ap.add_argument("--serve-jobs", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=0.0, help="arrivals per second; 0: 0.8 x measured capacity")
     ap.add_argument("--serve-estimator-precision", choices=["c64", "c128"], default="c64")
@@ -260,6 +262,10 @@ def main() -> int:
     info = H.device_info()
     log(f"device {info.name.decode()} {info.arch.decode()} CUs={info.compute_units} "
         f"HBM={info.total_bytes / 2**30:.0f} GiB free={info.free_bytes / 2**30:.0f} GiB")
+    if args.only:
+        section = {"batched": batched, "served": served, "complex64": complex64_report}[args.only]
+        print(json.dumps({args.only: section(args, H, W)}), flush=True)
+        return 0
     precision = H.C128 if args.precision == "c128" else H.C64
     amp = 16 if precision == H.C128 else 8
     fusion = {"none": H.FUSE_NONE, "containment": H.FUSE_CONTAINMENT, "width": H.FUSE_WIDTH}[args.fusion]
