@@ -144,11 +144,16 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
     double prepare_seconds = 0;
     {
         // Groups of items: the host plans and compiles group g + 1 while the
-        // device runs group g.
+        // device runs group g. Group sizes grow (1, 1, 2, 4, 8 sixteenths) so
+        // that only a sixteenth of the host work is exposed before the device
+        // starts, and every later group's host work is shorter than the device
+        // work queued before it.
         BatchPassRunner runner(precision, threads);
-        const std::size_t groups = count >= 64 ? 4 : 1;
+        static constexpr std::size_t kEnds[] = {1, 2, 4, 8, 16};
+        const std::size_t groups = count >= 64 ? 5 : 1;
         for (std::size_t g = 0; g < groups; ++g) {
-            const std::size_t lo = count * g / groups, hi = count * (g + 1) / groups;
+            const std::size_t lo = groups == 1 ? 0 : count * (g ? kEnds[g - 1] : 0) / 16;
+            const std::size_t hi = groups == 1 ? count : count * kEnds[g] / 16;
             if (prepare) {
                 const auto q0 = clock::now();
                 prepare(lo, hi);

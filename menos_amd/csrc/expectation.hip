@@ -60,41 +60,65 @@ __device__ __forceinline__ void expectation_group(const V* s, double* partials, 
 #pragma unroll
     for (uint32_t t = 0; t < T; ++t) accr[t] = acci[t] = 0.0;
 
+    // Items are taken kUnroll at a time: every load of a batch is issued
+    // before any is used (the loop is otherwise bound by memory latency), and
+    // the sums still run over the items in order.
+    constexpr uint32_t kUnroll = 4;
     if (x == 0) {
-        for (uint64_t m = tid; m < pairs; m += total_threads) {
-            double ar, ai;
-            load_amp(s, m, ar, ai);
-            // conj(a) * a: real ar*ar + ai*ai, imaginary ar*ai - ai*ar (exactly 0).
-            const double pr = ar * ar + ai * ai;
-            const double pi = ar * ai - ai * ar;
+        for (uint64_t m0 = tid; m0 < pairs; m0 += kUnroll * total_threads) {
+            double ar[kUnroll], ai[kUnroll];
 #pragma unroll
-            for (uint32_t t = 0; t < T; ++t) {
-                if (t >= terms) break;
-                const bool neg = (__popcll(z[t] & m) & 1) != 0;
-                accr[t] = neg ? accr[t] - pr : accr[t] + pr;
-                acci[t] = neg ? acci[t] - pi : acci[t] + pi;
+            for (uint32_t k = 0; k < kUnroll; ++k) {
+                const uint64_t m = m0 + k * total_threads;
+                if (m < pairs) load_amp(s, m, ar[k], ai[k]);
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < kUnroll; ++k) {
+                const uint64_t m = m0 + k * total_threads;
+                if (m >= pairs) break;
+                // conj(a) * a: real ar*ar + ai*ai, imaginary ar*ai - ai*ar (exactly 0).
+                const double pr = ar[k] * ar[k] + ai[k] * ai[k];
+                const double pi = ar[k] * ai[k] - ai[k] * ar[k];
+#pragma unroll
+                for (uint32_t t = 0; t < T; ++t) {
+                    if (t >= terms) break;
+                    const bool neg = (__popcll(z[t] & m) & 1) != 0;
+                    accr[t] = neg ? accr[t] - pr : accr[t] + pr;
+                    acci[t] = neg ? acci[t] - pi : acci[t] + pi;
+                }
             }
         }
     } else {
         const uint64_t lowm = (1ull << h) - 1ull;
-        for (uint64_t r = tid; r < pairs; r += total_threads) {
-            const uint64_t m = ((r & ~lowm) << 1) | (r & lowm);
-            const uint64_t mp = m ^ x;
-            double ar, ai, br, bi;
-            load_amp(s, m, ar, ai);
-            load_amp(s, mp, br, bi);
-            // p = conj(a) * b
-            const double pr = ar * br + ai * bi;
-            const double pi = ar * bi - ai * br;
+        for (uint64_t r0 = tid; r0 < pairs; r0 += kUnroll * total_threads) {
+            double ar[kUnroll], ai[kUnroll], br[kUnroll], bi[kUnroll];
 #pragma unroll
-            for (uint32_t t = 0; t < T; ++t) {
-                if (t >= terms) break;
-                const bool neg_mp = (__popcll(z[t] & mp) & 1) != 0;  // s(m') for p
-                const bool neg_m = (__popcll(z[t] & m) & 1) != 0;    // s(m) for conj(p)
-                accr[t] = neg_mp ? accr[t] - pr : accr[t] + pr;
-                acci[t] = neg_mp ? acci[t] - pi : acci[t] + pi;
-                accr[t] = neg_m ? accr[t] - pr : accr[t] + pr;
-                acci[t] = neg_m ? acci[t] + pi : acci[t] - pi;
+            for (uint32_t k = 0; k < kUnroll; ++k) {
+                const uint64_t r = r0 + k * total_threads;
+                if (r >= pairs) continue;
+                const uint64_t m = ((r & ~lowm) << 1) | (r & lowm);
+                load_amp(s, m, ar[k], ai[k]);
+                load_amp(s, m ^ x, br[k], bi[k]);
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < kUnroll; ++k) {
+                const uint64_t r = r0 + k * total_threads;
+                if (r >= pairs) break;
+                const uint64_t m = ((r & ~lowm) << 1) | (r & lowm);
+                const uint64_t mp = m ^ x;
+                // p = conj(a) * b
+                const double pr = ar[k] * br[k] + ai[k] * bi[k];
+                const double pi = ar[k] * bi[k] - ai[k] * br[k];
+#pragma unroll
+                for (uint32_t t = 0; t < T; ++t) {
+                    if (t >= terms) break;
+                    const bool neg_mp = (__popcll(z[t] & mp) & 1) != 0;  // s(m') for p
+                    const bool neg_m = (__popcll(z[t] & m) & 1) != 0;    // s(m) for conj(p)
+                    accr[t] = neg_mp ? accr[t] - pr : accr[t] + pr;
+                    acci[t] = neg_mp ? acci[t] - pi : acci[t] + pi;
+                    accr[t] = neg_m ? accr[t] - pr : accr[t] + pr;
+                    acci[t] = neg_m ? acci[t] + pi : acci[t] - pi;
+                }
             }
         }
     }
