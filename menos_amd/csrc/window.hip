@@ -99,6 +99,21 @@ struct Stages {
     static constexpr uint32_t reg_swap(uint32_t j1, uint32_t j2, uint32_t rb) { return 5 + kGroupSlots * rb + pair_index(j1, j2, rb); }
 };
 static_assert(Stages::reg_swap(2, 3, 4) < 64, "stages of one bundle fit 64 bits");
+
+// Every controlled-pair stage bit of a bundle (rb register bits).
+constexpr uint64_t pair_stage_bits(uint32_t rb) {
+    uint64_t m = 0;
+    for (uint32_t j = 0; j < rb; ++j)
+        for (uint32_t slot = 8; slot < Stages::kGroupSlots; ++slot) m |= 1ull << (Stages::group(j) + slot);
+    return m;
+}
+
+bool uses_pairs(const uint64_t* bundles, std::size_t count, uint32_t rb) {
+    const uint64_t m = pair_stage_bits(rb);
+    for (std::size_t b = 0; b < count; ++b)
+        if (bundles[b] & m) return true;
+    return false;
+}
 static_assert(Stages::pair_reg(0, 3) < Stages::group(1) && Stages::pair_reg(3, 2) < Stages::dense2(4),
               "a register bit's stages fit its group");
 
@@ -512,11 +527,11 @@ __device__ __forceinline__ void pair_reg_stage(uint64_t c, V* v, const WOp*& op,
 
 // The stages of register bit J (exchanges into J, then its 1-qubit blocks and
 // controlled pairs).
-template <typename V, uint32_t E, uint32_t RB, uint32_t J>
+template <typename V, uint32_t E, uint32_t RB, uint32_t J, bool kPairs>
 __device__ __forceinline__ void group_stages(uint64_t c, V* v, const WOp*& op, const V* mats, uint64_t obase,
                                              uint32_t lane) {
     if constexpr (J < RB) {
-        if (!(c & (0xFFFull << Stages::group(J)))) return;
+        if (!(c & ((kPairs ? 0xFFFull : 0xFFull) << Stages::group(J)))) return;
         if (c & (1ull << Stages::lane_swap(0, J))) lane_swap<V, E, 0, J>(v, lane);
         if (c & (1ull << Stages::lane_swap(1, J))) lane_swap<V, E, 1, J>(v, lane);
         if (c & (1ull << Stages::lane_swap(2, J))) lane_swap<V, E, 2, J>(v, lane);
@@ -525,6 +540,7 @@ __device__ __forceinline__ void group_stages(uint64_t c, V* v, const WOp*& op, c
         if (c & (1ull << Stages::lane_swap(5, J))) lane_swap<V, E, 5, J>(v, lane);
         if (c & (1ull << Stages::dense1_re(J))) dense1<E, J, true>(v, mats + (op++)->mat);
         if (c & (1ull << Stages::dense1(J))) dense1<E, J, false>(v, mats + (op++)->mat);
+        if constexpr (!kPairs) return;
         if (c & (1ull << Stages::pair_var(J))) {
             const WOp& o = *op++;
             const bool set = o.lane_bit[0] != kNone ? ((lane >> o.lane_bit[0]) & 1u) != 0
@@ -546,7 +562,7 @@ __device__ __forceinline__ void reg_swap_stage(uint64_t c, V* v) {
 }
 
 // One wave's window of one pass.
-template <typename R>
+template <typename R, bool kPairs>
 __device__ __forceinline__ void window_body(typename Shape<R>::V* state, const uint64_t* bundles, const WOp* ops,
                                             const SignTerm* terms, const typename Shape<R>::V* mats,
                                             const WLayout& a, uint32_t wave, uint32_t lane) {
@@ -577,11 +593,11 @@ __device__ __forceinline__ void window_body(typename Shape<R>::V* state, const u
         const uint64_t c = bundles[b];
         if (c & (1ull << Stages::kSignEarly)) apply_signs<V, E>(v, *op++, terms, obase, lane);
         if (c & (1ull << Stages::kDiagEarly)) apply_diag<V, E>(v, *op++, mats, obase, lane);
-        group_stages<V, E, RB, 0>(c, v, op, mats, obase, lane);
-        group_stages<V, E, RB, 1>(c, v, op, mats, obase, lane);
-        group_stages<V, E, RB, 2>(c, v, op, mats, obase, lane);
-        group_stages<V, E, RB, 3>(c, v, op, mats, obase, lane);
-        group_stages<V, E, RB, 4>(c, v, op, mats, obase, lane);
+        group_stages<V, E, RB, 0, kPairs>(c, v, op, mats, obase, lane);
+        group_stages<V, E, RB, 1, kPairs>(c, v, op, mats, obase, lane);
+        group_stages<V, E, RB, 2, kPairs>(c, v, op, mats, obase, lane);
+        group_stages<V, E, RB, 3, kPairs>(c, v, op, mats, obase, lane);
+        group_stages<V, E, RB, 4, kPairs>(c, v, op, mats, obase, lane);
         if (c & (1ull << Stages::dense2(RB))) dense2<E>(v, mats + (op++)->mat);
         if (c & (1ull << Stages::sign_late(RB))) apply_signs<V, E>(v, *op++, terms, obase, lane);
         if (c & (1ull << Stages::diag_late(RB))) apply_diag<V, E>(v, *op++, mats, obase, lane);
@@ -627,18 +643,20 @@ __device__ __forceinline__ void window_body(typename Shape<R>::V* state, const u
 }
 
 // Four waves per SIMD: each wave holds its window in at most 128 VGPRs.
-template <typename R>
+// kPairs: the pass has controlled-pair stages (the form without them is the
+// leaner code for passes that need none).
+template <typename R, bool kPairs>
 __global__ void __launch_bounds__(kWavesPerGroup * 64, 4) window_kernel(WPassArgs a) {
     using V = typename Shape<R>::V;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-    window_body<R>(static_cast<V*>(a.state), a.bundles, a.ops, a.terms, static_cast<const V*>(a.mats), a.lay, wave,
-                   threadIdx.x & 63u);
+    window_body<R, kPairs>(static_cast<V*>(a.state), a.bundles, a.ops, a.terms, static_cast<const V*>(a.mats), a.lay,
+                           wave, threadIdx.x & 63u);
 }
 
 // One pass of many states in one launch: workgroup b belongs to the state c
 // with group_begin[c] <= b < group_begin[c + 1] (each state's windows in
 // groups of four waves); a wave past its state's last window returns.
-template <typename R>
+template <typename R, bool kPairs>
 __global__ void __launch_bounds__(kWavesPerGroup * 64, 4)
     window_batch_kernel(const WBatchArgs* __restrict__ args, const uint32_t* __restrict__ group_begin, uint32_t count,
                         typename Shape<R>::V* state, const uint64_t* __restrict__ bundles,
@@ -658,8 +676,8 @@ __global__ void __launch_bounds__(kWavesPerGroup * 64, 4)
     const uint32_t wave =
         __builtin_amdgcn_readfirstlane((b - group_begin[lo]) * kWavesPerGroup + (threadIdx.x >> 6));
     if (wave >= a.lay.num_waves) return;
-    window_body<R>(state + a.state_off, bundles + a.bundle_off, ops + a.op_off, terms + a.term_off,
-                   mats + a.mat_off, a.lay, wave, threadIdx.x & 63u);
+    window_body<R, kPairs>(state + a.state_off, bundles + a.bundle_off, ops + a.op_off, terms + a.term_off,
+                           mats + a.mat_off, a.lay, wave, threadIdx.x & 63u);
 }
 
 // ---- host: passes
@@ -1146,7 +1164,10 @@ void launch_window_passes(void* state, uint32_t n, const std::vector<Block>& blo
             const uint32_t waves_log = n - W;
             const uint32_t group_log = std::min<uint32_t>(waves_log, 2);
             const dim3 grid(1u << (waves_log - group_log)), block(64u << group_log);
-            hipLaunchKernelGGL(window_kernel<R>, grid, block, 0, ctx.stream, a);
+            if (uses_pairs(bundles.data() + bundle_begin[p], args[p].lay.num_bundles, Shape<R>::kRegBits))
+                hipLaunchKernelGGL((window_kernel<R, true>), grid, block, 0, ctx.stream, a);
+            else
+                hipLaunchKernelGGL((window_kernel<R, false>), grid, block, 0, ctx.stream, a);
             MH_HIP(hipGetLastError());
         }
         if (time_passes) MH_HIP(hipEventRecord(events[p + 1], ctx.stream));
@@ -1286,6 +1307,7 @@ void launch_window_passes_batch(const BatchItem* items, std::size_t count, uint3
     std::vector<uint32_t> all_groups;
     std::vector<std::size_t> args_at(max_passes), groups_at(max_passes), list_len(max_passes);
     std::vector<uint32_t> total_groups(max_passes);
+    std::vector<char> pairs_in(max_passes, 0);
     for (std::size_t p = 0; p < max_passes; ++p) {
         args_at[p] = all_args.size();
         groups_at[p] = all_groups.size();
@@ -1303,6 +1325,8 @@ void launch_window_passes_batch(const BatchItem* items, std::size_t count, uint3
             a.term_off = static_cast<uint32_t>(term_base[i]);
             a.mat_off = static_cast<uint32_t>(mat_base[i]);
             a.lay = rec.args.lay;
+            if (uses_pairs(comp[i].bundles.data() + rec.bundle_begin, rec.args.lay.num_bundles, Shape<R>::kRegBits))
+                pairs_in[p] = 1;
             all_args.push_back(a);
             all_groups.push_back(g);
             g += (a.lay.num_waves + kWavesPerGroup - 1) / kWavesPerGroup;
@@ -1330,9 +1354,14 @@ void launch_window_passes_batch(const BatchItem* items, std::size_t count, uint3
     uint32_t launches = 0;
     for (std::size_t p = 0; p < max_passes; ++p) {
         if (list_len[p] > 0) {
-            hipLaunchKernelGGL(window_batch_kernel<R>, dim3(total_groups[p]), dim3(kWavesPerGroup * 64), 0, ctx.stream,
-                               dargs_p + args_at[p], dgroups_p + groups_at[p], static_cast<uint32_t>(list_len[p]),
-                               state_base, dbundles_p, dops_p, dterms_p, dmats_p);
+            if (pairs_in[p])
+                hipLaunchKernelGGL((window_batch_kernel<R, true>), dim3(total_groups[p]), dim3(kWavesPerGroup * 64), 0,
+                                   ctx.stream, dargs_p + args_at[p], dgroups_p + groups_at[p],
+                                   static_cast<uint32_t>(list_len[p]), state_base, dbundles_p, dops_p, dterms_p, dmats_p);
+            else
+                hipLaunchKernelGGL((window_batch_kernel<R, false>), dim3(total_groups[p]), dim3(kWavesPerGroup * 64), 0,
+                                   ctx.stream, dargs_p + args_at[p], dgroups_p + groups_at[p],
+                                   static_cast<uint32_t>(list_len[p]), state_base, dbundles_p, dops_p, dterms_p, dmats_p);
             MH_HIP(hipGetLastError());
             ++launches;
         }
