@@ -134,14 +134,16 @@ struct WOp {
     uint8_t rpart[32];       // diagonal: operand bits that register index r supplies
 };
 
-// One sign diagonal of a sign stage: where each operand's bit comes from.
+// One -1 entry of a sign diagonal in a sign stage, resolved on the host
+// against where each operand lives: the amplitudes it negates are those whose
+// outside-window qubits, lane bits and register bits all match the entry.
 struct SignTerm {
-    uint64_t neg_mask;       // bit e set when entry e is -1
-    uint8_t k;
-    uint8_t src[6];          // operand u: lane bit 0..5, register bit 8 + j, or 16 (outside the window)
-    uint8_t q[6];            // outside the window: the global qubit
-    uint8_t pad[3];
+    uint64_t omask, oval;    // outside the window: (wave base & omask) == oval
+    uint32_t rpat;           // register indices r that match (bit r)
+    uint8_t lmask, lval;     // lane bits: (lane & lmask) == lval
+    uint8_t pad[2];
 };
+static_assert(sizeof(SignTerm) == 24, "sign term layout");
 
 // Where a pass's window qubits live at the load and the store, and which
 // outside qubits the wave index selects.
@@ -464,42 +466,24 @@ __device__ __forceinline__ uint32_t diag_base(const WOp& op, uint64_t obase, uin
     return t;
 }
 
-// Register index bits of register bit j (bit r set when bit j of r is).
-__constant__ const uint32_t kRegPattern[5] = {0xAAAAAAAAu, 0xCCCCCCCCu, 0xF0F0F0F0u, 0xFF00FF00u, 0xFFFF0000u};
-
 // A run of sign diagonals: bit r of m tells whether amplitude r changes sign.
-// Each term adds (XOR) the indicator of its -1 entries, a sum of minterms
-// over its operand bits; an operand bit as a mask over r is a constant for a
-// register bit and all-or-nothing for a lane bit or a qubit outside the window.
+// Each -1 entry of each diagonal adds (XOR) its register pattern where its
+// outside-window qubits (the same for the whole wave) and lane bits match:
+// entries that name no lane bit are one scalar test each.
 template <typename V, uint32_t E>
 __device__ __forceinline__ void apply_signs(V* v, const WOp& op, const SignTerm* terms, uint64_t obase,
                                             uint32_t lane) {
     constexpr uint32_t full = E == 32 ? 0xFFFFFFFFu : (1u << E) - 1u;
-    uint32_t m = 0;
+    uint32_t mu = 0, ml = 0;
     for (uint32_t t = 0; t < op.num_outer; ++t) {
         const SignTerm& st = terms[op.mat + t];
-        uint32_t a[6];
-#pragma unroll
-        for (uint32_t u = 0; u < 6; ++u) {
-            a[u] = 0;
-            if (u >= st.k) continue;
-            const uint32_t src = st.src[u];
-            if (src < 8)
-                a[u] = ((lane >> src) & 1u) ? full : 0u;
-            else if (src < 16)
-                a[u] = kRegPattern[src - 8] & full;
-            else
-                a[u] = ((obase >> st.q[u]) & 1ull) ? full : 0u;
-        }
-        for (uint64_t nm = st.neg_mask; nm != 0; nm &= nm - 1) {
-            const uint32_t e = static_cast<uint32_t>(__builtin_ctzll(nm));
-            uint32_t mt = full;
-#pragma unroll
-            for (uint32_t u = 0; u < 6; ++u)
-                if (u < st.k) mt &= ((e >> u) & 1u) ? a[u] : ~a[u];
-            m ^= mt;
-        }
+        if ((obase & st.omask) != st.oval) continue;
+        if (st.lmask == 0)
+            mu ^= st.rpat;
+        else
+            ml ^= (lane & st.lmask) == st.lval ? st.rpat : 0u;
     }
+    const uint32_t m = (mu ^ ml) & full;
 #pragma unroll
     for (uint32_t r = 0; r < E; ++r) flip_sign(v[r], (m >> r) & 1u);
 }
@@ -756,12 +740,6 @@ uint64_t register_mask(const Block& b) {
     const int u = pair_control(b);
     for (uint32_t t = 0; t < b.operands.size(); ++t)
         if (static_cast<int>(t) != u) m |= 1ull << b.operands[t];
-    return m;
-}
-
-uint64_t mask_of(const std::vector<uint32_t>& q) {
-    uint64_t m = 0;
-    for (uint32_t x : q) m |= 1ull << x;
     return m;
 }
 
@@ -1063,37 +1041,47 @@ private:
         put(Stages::kDiagEarly, static_cast<int>(Stages::diag_late(RB)), &op);
     }
 
-    // A sign diagonal joins the sign stage just emitted, if nothing came after it.
+    // A sign diagonal joins the sign stage just emitted, if nothing came after
+    // it: one term per -1 entry, resolved against the current layout.
     void emit_sign(const Block& b) {
         const uint32_t k = static_cast<uint32_t>(b.operands.size());
         const std::size_t dim = std::size_t{1} << k;
-        SignTerm t;
-        std::memset(&t, 0, sizeof(t));
-        t.k = static_cast<uint8_t>(k);
-        for (uint32_t u = 0; u < k; ++u) {
-            const uint32_t q = b.operands[u];
-            const uint8_t p = pos_of_[q];
-            if (p == kNone) {
-                t.src[u] = 16;
-                t.q[u] = static_cast<uint8_t>(q);
-            } else if (lane_index(p) >= 0) {
-                t.src[u] = static_cast<uint8_t>(lane_index(p));
-            } else {
-                t.src[u] = static_cast<uint8_t>(8 + reg_index(p));
+        const uint32_t full = E == 32 ? 0xFFFFFFFFu : (1u << E) - 1u;
+        uint32_t added = 0;
+        for (std::size_t e = 0; e < dim; ++e) {
+            if (b.matrix[e * dim + e].real() != -1.0) continue;
+            SignTerm t;
+            std::memset(&t, 0, sizeof(t));
+            t.rpat = full;
+            for (uint32_t u = 0; u < k; ++u) {
+                const uint32_t q = b.operands[u];
+                const bool one = (e >> u) & 1u;
+                const uint8_t p = pos_of_[q];
+                if (p == kNone) {
+                    t.omask |= 1ull << q;
+                    if (one) t.oval |= 1ull << q;
+                } else if (const int li = lane_index(p); li >= 0) {
+                    t.lmask |= static_cast<uint8_t>(1u << li);
+                    if (one) t.lval |= static_cast<uint8_t>(1u << li);
+                } else {
+                    const int j = reg_index(p);
+                    uint32_t pat = 0;
+                    for (uint32_t r = 0; r < E; ++r)
+                        if ((r >> j) & 1u) pat |= 1u << r;
+                    t.rpat &= one ? pat : ~pat;
+                }
             }
-        }
-        for (std::size_t e = 0; e < dim; ++e)
-            if (b.matrix[e * dim + e].real() == -1.0) t.neg_mask |= 1ull << e;
-        if (open_sign_ >= 0) {
             terms_.push_back(t);
-            ++ops_[static_cast<std::size_t>(open_sign_)].num_outer;
+            ++added;
+        }
+        if (open_sign_ >= 0) {
+            ops_[static_cast<std::size_t>(open_sign_)].num_outer += added;
             return;
         }
         WOp op;
         std::memset(&op, 0, sizeof(op));
-        op.mat = static_cast<uint32_t>(terms_.size());
-        op.num_outer = 1;
-        terms_.push_back(t);
+        op.mat = static_cast<uint32_t>(terms_.size()) - added;
+        op.num_outer = added;
         put(Stages::kSignEarly, static_cast<int>(Stages::sign_late(RB)), &op);
         open_sign_ = static_cast<int>(ops_.size()) - 1;
     }
