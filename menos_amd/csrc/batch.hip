@@ -124,6 +124,15 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
     // Initialise: zeros, then each state's first amplitude.
     MH_HIP(hipMemsetAsync(arena, 0, bytes, ctx.stream));
     DeviceBuffer doffsets(offsets.size() * sizeof(uint64_t)), dphases(phases.size() * sizeof(double));
+    // On an error past this point, wait for the queued work before the
+    // buffers it uses go back to the pool.
+    struct WaitOnUnwind {
+        hipStream_t stream;
+        int pending = std::uncaught_exceptions();
+        ~WaitOnUnwind() {
+            if (std::uncaught_exceptions() > pending) (void)hipStreamSynchronize(stream);
+        }
+    } wait_on_unwind{ctx.stream};
     upload(doffsets.get(), offsets.data(), offsets.size() * sizeof(uint64_t));
     upload(dphases.get(), phases.data(), phases.size() * sizeof(double));
     const uint32_t n_here = static_cast<uint32_t>(count);
