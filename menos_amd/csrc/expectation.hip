@@ -5,14 +5,18 @@
 //
 //     i^popcount(x & z) * sum_m conj(psi[m]) * psi[m ^ x] * (-1)^popcount(z & (m ^ x)).
 //
-// Terms sharing an X mask share their products, so one launch reads the state
-// once for up to kTermsPerLaunch such terms. For x != 0 the sum is taken over
+// Terms sharing an X mask share their products. X masks whose bits fit an LDS
+// tile (with the lowest qubits) are reduced from tiles: one read of the state
+// serves a whole set of X masks (expectation_tile_kernel). A wider X mask reads
+// the state once for up to kTermsPerLaunch terms (expectation_group). For x != 0
+// the sum is taken over
 // pairs (m, m' = m ^ x) with bit h = msb(x) of m clear: with p = conj(psi[m]) *
 // psi[m'], the pair contributes p * s(m') + conj(p) * s(m), s(k) =
 // (-1)^popcount(z & k); conj(p) is bitwise conj(psi[m']) * psi[m], so each
-// product is the reference's, read once. Each thread sums a fixed grid-stride
-// set of items in order, then the workgroup and the partials are added in a fixed
-// tree order: the value is reproducible run to run and differs from the
+// product is the reference's, read once. Each thread sums a fixed set of items
+// in order, then the workgroup and the partials are added in a fixed tree order:
+// the value is reproducible run to run, the same for one state alone and in a
+// batch (both take this path), and differs from the
 // reference's sequential index-order sum by rounding only (the reference's
 // own CUDA contract, dense_kernels.h:237-246).
 #include <algorithm>
@@ -28,17 +32,6 @@ namespace {
 constexpr int kThreads = 256;
 constexpr uint32_t kTermsPerLaunch = 16;
 constexpr uint32_t kMaxBlocks = 2048;
-
-struct ExpArgs {
-    const void* state;
-    double* partials;   // [term][block] complex as 2 doubles
-    uint64_t pairs;     // work items: 2^n (x == 0) or 2^(n-1)
-    uint64_t x;
-    uint32_t h;         // msb of x (unused when x == 0)
-    uint32_t terms;
-    uint32_t blocks;
-    uint64_t z[kTermsPerLaunch];
-};
 
 template <typename V>
 __device__ __forceinline__ void load_amp(const V* s, uint64_t i, double& re, double& im) {
@@ -146,12 +139,6 @@ __device__ __forceinline__ void expectation_group(const V* s, double* partials, 
     }
 }
 
-template <typename V, uint32_t T>
-__global__ void __launch_bounds__(kThreads) expectation_kernel(ExpArgs a) {
-    expectation_group<V, T>(static_cast<const V*>(a.state), a.partials, a.pairs, a.x, a.h, a.terms, a.z, blockIdx.x,
-                            a.blocks);
-}
-
 // One group of terms of one state in a batched launch; offsets count from the
 // launch's base pointers (amplitudes, doubles).
 struct ExpJob {
@@ -181,6 +168,146 @@ __global__ void __launch_bounds__(kThreads)
     const ExpJob& j = jobs[lo];
     expectation_group<V, kTermsPerLaunch>(base + j.state_off, partials + j.partial_off, j.pairs, j.x, j.h, j.terms,
                                           j.z, b - block_begin[lo], j.blocks);
+}
+
+// Tiles: a workgroup loads 2^k amplitudes of one state into LDS at a time
+// (the tile bits T: every X bit of a set of up to kTermsPerLaunch terms, the
+// lowest qubits for whole 128-byte runs, then the next lowest), and reduces
+// every term of the set from there, so the state is read from HBM once per set
+// rather than once per X mask. Every (m, m ^ x) pair of a term lies in one
+// tile because x's bits are in T. A workgroup takes tiles b, b + blocks, ...
+// in order and keeps one sum per term across them.
+constexpr uint32_t kTileBits = 11;
+
+struct TileJob {
+    uint64_t state_off;      // amplitudes from the launch base
+    uint64_t outer_mask;     // index bits outside T: the tile index deposits here, ascending
+    uint32_t k;              // tile bits (2^k amplitudes per tile)
+    uint32_t term_begin, term_count;
+    uint32_t tiles, blocks;
+    uint32_t pad;
+    uint8_t tpos[16];        // tile bit i -> qubit, ascending
+};
+
+struct TileTerm {
+    uint64_t x, z;
+    uint32_t xl;             // x on the tile bits
+    uint32_t hl;             // msb of xl (x != 0)
+    uint64_t partial_off;    // partials[(block) * 2 + {0, 1}] from here
+};
+
+template <typename V>
+__global__ void __launch_bounds__(kThreads)
+    expectation_tile_kernel(const TileJob* __restrict__ jobs, const uint32_t* __restrict__ block_begin,
+                            uint32_t count, const TileTerm* __restrict__ terms, const V* base, double* partials) {
+    __shared__ V tile[1u << kTileBits];
+    __shared__ uint64_t dep_lo[64], dep_hi[32];
+    __shared__ double red[2][kThreads];
+    const uint32_t b = blockIdx.x;
+    uint32_t lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (block_begin[mid] <= b) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    const TileJob& job = jobs[lo];
+    const uint32_t blk = b - block_begin[lo];
+    const uint32_t k = job.k, size = 1u << k;
+    // Tile-local index -> global index bits, as two lookup tables (bits 0-5, 6-10).
+    if (threadIdx.x < 96) {
+        const uint32_t part = threadIdx.x < 64 ? threadIdx.x : (threadIdx.x - 64) << 6;
+        uint64_t g = 0;
+        for (uint32_t i = 0; i < k; ++i)
+            if ((part >> i) & 1u) g |= 1ull << job.tpos[i];
+        if (threadIdx.x < 64)
+            dep_lo[threadIdx.x] = g;
+        else
+            dep_hi[threadIdx.x - 64] = g;
+    }
+    const V* s = base + job.state_off;
+    const uint32_t nt = job.term_count;
+    double accr[kTermsPerLaunch], acci[kTermsPerLaunch];
+#pragma unroll
+    for (uint32_t t = 0; t < kTermsPerLaunch; ++t) accr[t] = acci[t] = 0.0;
+
+    for (uint32_t tl = blk; tl < job.tiles; tl += job.blocks) {
+        uint64_t tbase = 0;
+        {
+            uint64_t om = job.outer_mask;
+            for (uint32_t t = tl; om != 0; t >>= 1) {
+                const uint64_t low = om & (~om + 1);
+                if (t & 1u) tbase |= low;
+                om ^= low;
+            }
+        }
+        __syncthreads();  // the tables are written; the previous tile is no longer read
+        for (uint32_t j = threadIdx.x; j < size; j += kThreads) tile[j] = s[tbase | dep_lo[j & 63u] | dep_hi[j >> 6]];
+        __syncthreads();
+#pragma unroll
+        for (uint32_t t = 0; t < kTermsPerLaunch; ++t) {
+            if (t >= nt) continue;
+            const TileTerm& tt = terms[job.term_begin + t];
+            if (tt.x == 0) {
+                for (uint32_t j = threadIdx.x; j < size; j += kThreads) {
+                    const uint64_t m = tbase | dep_lo[j & 63u] | dep_hi[j >> 6];
+                    const double ar = static_cast<double>(tile[j].x), ai = static_cast<double>(tile[j].y);
+                    const double pr = ar * ar + ai * ai;
+                    const double pi = ar * ai - ai * ar;
+                    const bool neg = (__popcll(tt.z & m) & 1) != 0;
+                    accr[t] = neg ? accr[t] - pr : accr[t] + pr;
+                    acci[t] = neg ? acci[t] - pi : acci[t] + pi;
+                }
+            } else {
+                const uint32_t lowm = (1u << tt.hl) - 1u;
+                for (uint32_t r = threadIdx.x; r < size / 2; r += kThreads) {
+                    const uint32_t j = ((r & ~lowm) << 1) | (r & lowm);
+                    const uint32_t jp = j ^ tt.xl;
+                    const uint64_t m = tbase | dep_lo[j & 63u] | dep_hi[j >> 6];
+                    const uint64_t mp = m ^ tt.x;
+                    const double ar = static_cast<double>(tile[j].x), ai = static_cast<double>(tile[j].y);
+                    const double br = static_cast<double>(tile[jp].x), bi = static_cast<double>(tile[jp].y);
+                    // p = conj(a) * b, with a at m (bit h of m clear) and b at m ^ x
+                    const double pr = ar * br + ai * bi;
+                    const double pi = ar * bi - ai * br;
+                    const bool neg_mp = (__popcll(tt.z & mp) & 1) != 0;
+                    const bool neg_m = (__popcll(tt.z & m) & 1) != 0;
+                    accr[t] = neg_mp ? accr[t] - pr : accr[t] + pr;
+                    acci[t] = neg_mp ? acci[t] - pi : acci[t] + pi;
+                    accr[t] = neg_m ? accr[t] - pr : accr[t] + pr;
+                    acci[t] = neg_m ? acci[t] + pi : acci[t] - pi;
+                }
+            }
+        }
+    }
+    // Fixed-order tree over the workgroup, one term at a time.
+    for (uint32_t t = 0; t < nt && t < kTermsPerLaunch; ++t) {
+        double vr = 0.0, vi = 0.0;
+#pragma unroll
+        for (uint32_t u = 0; u < kTermsPerLaunch; ++u)
+            if (u == t) {
+                vr = accr[u];
+                vi = acci[u];
+            }
+        red[0][threadIdx.x] = vr;
+        red[1][threadIdx.x] = vi;
+        __syncthreads();
+        for (int stride = kThreads / 2; stride > 0; stride >>= 1) {
+            if (threadIdx.x < stride) {
+                red[0][threadIdx.x] = red[0][threadIdx.x] + red[0][threadIdx.x + stride];
+                red[1][threadIdx.x] = red[1][threadIdx.x] + red[1][threadIdx.x + stride];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            double* pp = partials + terms[job.term_begin + t].partial_off + uint64_t(blk) * 2;
+            pp[0] = red[0][0];
+            pp[1] = red[1][0];
+        }
+        __syncthreads();
+    }
 }
 
 // One term slot of a batched launch.
@@ -222,12 +349,6 @@ __device__ __forceinline__ void finish_term(const double* partials, uint32_t blo
 }
 
 // One workgroup per term.
-__global__ void __launch_bounds__(kThreads) expectation_finish_kernel(const double* partials, uint32_t blocks,
-                                                                      const int32_t* phase_k, double* out) {
-    const uint32_t t = blockIdx.x;
-    finish_term(partials + uint64_t(t) * blocks * 2, blocks, phase_k[t], out + 2 * t);
-}
-
 __global__ void __launch_bounds__(kThreads) expectation_finish_batch_kernel(const ExpSlot* __restrict__ slots,
                                                                             const double* partials, double* out) {
     const ExpSlot& s = slots[blockIdx.x];
@@ -238,70 +359,24 @@ uint32_t blocks_for(uint64_t pairs) {
     return static_cast<uint32_t>(std::min<uint64_t>(kMaxBlocks, std::max<uint64_t>(1, pairs / (kThreads * 16))));
 }
 
-template <typename V>
-void run_group(const void* state, uint32_t n, uint64_t x, const std::vector<uint32_t>& idx, const uint64_t* z,
-               double* out_all) {
-    Context& ctx = context();
-    const uint32_t terms = static_cast<uint32_t>(idx.size());
-    ExpArgs a{};
-    a.state = state;
-    a.x = x;
-    a.h = x ? 63u - static_cast<uint32_t>(__builtin_clzll(x)) : 0u;
-    a.pairs = x ? (uint64_t{1} << (n - 1)) : (uint64_t{1} << n);
-    a.terms = terms;
-    a.blocks = blocks_for(a.pairs);
-    for (uint32_t t = 0; t < terms; ++t) a.z[t] = z[idx[t]];
-    DeviceBuffer partials(uint64_t(a.blocks) * terms * 2 * sizeof(double));
-    DeviceBuffer res(uint64_t(terms) * 2 * sizeof(double) + terms * sizeof(int32_t));
-    std::vector<int32_t> phase(terms);
-    for (uint32_t t = 0; t < terms; ++t) phase[t] = __builtin_popcountll(x & a.z[t]);
-    int32_t* dphase = reinterpret_cast<int32_t*>(res.as<double>() + 2 * terms);
-    upload(dphase, phase.data(), terms * sizeof(int32_t));
-    a.partials = partials.as<double>();
-    if (terms <= 1)
-        hipLaunchKernelGGL((expectation_kernel<V, 1>), dim3(a.blocks), dim3(kThreads), 0, ctx.stream, a);
-    else if (terms <= 4)
-        hipLaunchKernelGGL((expectation_kernel<V, 4>), dim3(a.blocks), dim3(kThreads), 0, ctx.stream, a);
-    else
-        hipLaunchKernelGGL((expectation_kernel<V, kTermsPerLaunch>), dim3(a.blocks), dim3(kThreads), 0, ctx.stream, a);
-    MH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(expectation_finish_kernel, dim3(terms), dim3(kThreads), 0, ctx.stream,
-                       static_cast<const double*>(partials.get()), a.blocks, static_cast<const int32_t*>(dphase),
-                       res.as<double>());
-    MH_HIP(hipGetLastError());
-    std::vector<double> host(2 * terms);
-    download(host.data(), res.get(), host.size() * sizeof(double));
-    for (uint32_t t = 0; t < terms; ++t) {
-        out_all[2 * idx[t]] = host[2 * t];
-        out_all[2 * idx[t] + 1] = host[2 * t + 1];
-    }
-}
-
 }  // namespace
 
 void expectation_terms(const void* state, uint32_t n, int32_t precision, const uint64_t* x, const uint64_t* z,
                        uint32_t terms, cplx* out) {
-    amp_bytes(precision);
-    if (n == 0 || n > 40) fail(MH_ERR_INVALID, "expectation: num_qubits must be 1..40");
-    const uint64_t limit = n == 64 ? ~0ull : ((uint64_t{1} << n) - 1);
-    std::map<uint64_t, std::vector<uint32_t>> groups;
-    for (uint32_t t = 0; t < terms; ++t) {
-        if ((x[t] & ~limit) || (z[t] & ~limit)) fail(MH_ERR_INVALID, "a Pauli mask names a qubit beyond the state");
-        groups[x[t]].push_back(t);
-    }
-    double* o = reinterpret_cast<double*>(out);
-    for (auto& [xm, members] : groups) {
-        for (std::size_t at = 0; at < members.size(); at += kTermsPerLaunch) {
-            std::vector<uint32_t> part(members.begin() + at,
-                                       members.begin() + std::min(members.size(), at + kTermsPerLaunch));
-            if (precision == MH_C128)
-                run_group<double2>(state, n, xm, part, z, o);
-            else
-                run_group<float2>(state, n, xm, part, z, o);
-        }
-    }
+    ExpectationItem it;
+    it.state = state;
+    it.n = n;
+    it.terms = terms;
+    it.x = x;
+    it.z = z;
+    it.out_index = 0;
+    expectation_batch(precision, {it}, reinterpret_cast<double*>(out));
 }
 
+// Every item's terms in two or three launches: X masks whose bits fit a tile
+// (with the lowest qubits) reduce from LDS tiles, a set of them per state read;
+// any other X mask (more than kTileBits - 3 bits) on the per-mask kernel; then
+// one finishing launch over every term.
 template <typename V>
 void expectation_batch_typed(const std::vector<ExpectationItem>& items, double* out) {
     Context& ctx = context();
@@ -309,9 +384,26 @@ void expectation_batch_typed(const std::vector<ExpectationItem>& items, double* 
     for (const ExpectationItem& it : items) base_addr = std::min(base_addr, reinterpret_cast<uintptr_t>(it.state));
     std::vector<ExpJob> jobs;
     std::vector<uint32_t> block_begin;
+    std::vector<TileJob> tjobs;
+    std::vector<TileTerm> tterms;
+    std::vector<uint32_t> tblock_begin;
     std::vector<ExpSlot> slots;
     uint64_t partial_doubles = 0;
-    uint32_t total_blocks = 0;
+    uint32_t total_blocks = 0, tile_blocks = 0;
+    auto add_slots = [&](uint64_t xm, const uint64_t* zs, const std::vector<uint32_t>& members, std::size_t at,
+                         std::size_t cnt, uint32_t blocks, uint32_t out_index) {
+        for (std::size_t k = 0; k < cnt; ++k) {
+            const uint32_t t = members[at + k];
+            ExpSlot sl;
+            std::memset(&sl, 0, sizeof(sl));
+            sl.partial_off = partial_doubles + uint64_t(k) * blocks * 2;
+            sl.blocks = blocks;
+            sl.phase_k = __builtin_popcountll(xm & zs[t]);
+            sl.out = out_index + t;
+            slots.push_back(sl);
+        }
+        partial_doubles += uint64_t(cnt) * blocks * 2;
+    };
     for (const ExpectationItem& it : items) {
         if (it.n == 0 || it.n > 40) fail(MH_ERR_INVALID, "expectation: num_qubits must be 1..40");
         const uint64_t limit = (uint64_t{1} << it.n) - 1;
@@ -321,49 +413,124 @@ void expectation_batch_typed(const std::vector<ExpectationItem>& items, double* 
             groups[it.x[t]].push_back(t);
         }
         const uintptr_t addr = reinterpret_cast<uintptr_t>(it.state);
-        for (auto& [xm, members] : groups) {
+        const uint64_t state_off = (addr - base_addr) / sizeof(V);
+        const uint32_t k = std::min<uint32_t>(kTileBits, it.n);
+        const uint64_t low = (uint64_t{1} << std::min<uint32_t>(3, k)) - 1;
+        // Sets of up to kTermsPerLaunch terms whose X bits, with the low run,
+        // fit k tile bits (terms taken in X-mask order).
+        std::vector<std::vector<uint32_t>> sets;
+        std::vector<uint64_t> wide;
+        uint64_t uni = 0;
+        for (const auto& [xm, members] : groups) {
+            if (static_cast<uint32_t>(__builtin_popcountll(xm | low)) > k) {
+                wide.push_back(xm);
+                continue;
+            }
+            for (uint32_t t : members) {
+                if (sets.empty() || sets.back().size() >= kTermsPerLaunch ||
+                    static_cast<uint32_t>(__builtin_popcountll(uni | xm | low)) > k) {
+                    sets.emplace_back();
+                    uni = low;
+                }
+                uni |= xm;
+                sets.back().push_back(t);
+            }
+        }
+        const uint32_t tiles = static_cast<uint32_t>(uint64_t{1} << (it.n - k));
+        // About eight tiles per workgroup for large states, one for small ones.
+        const uint32_t tile_blocks_each = std::max<uint32_t>(1, std::min<uint32_t>(tiles, std::max<uint32_t>(tiles / 8, 1)));
+        for (const std::vector<uint32_t>& set : sets) {
+            uint64_t tmask = low;
+            for (uint32_t t : set) tmask |= it.x[t];
+            for (uint32_t q = 0; q < it.n && static_cast<uint32_t>(__builtin_popcountll(tmask)) < k; ++q)
+                tmask |= uint64_t{1} << q;
+            TileJob tj;
+            std::memset(&tj, 0, sizeof(tj));
+            tj.state_off = state_off;
+            tj.outer_mask = limit & ~tmask;
+            tj.k = k;
+            tj.tiles = tiles;
+            tj.blocks = tile_blocks_each;
+            uint32_t at_bit = 0;
+            for (uint32_t q = 0; q < it.n; ++q)
+                if ((tmask >> q) & 1ull) tj.tpos[at_bit++] = static_cast<uint8_t>(q);
+            tj.term_begin = static_cast<uint32_t>(tterms.size());
+            tj.term_count = static_cast<uint32_t>(set.size());
+            for (uint32_t t : set) {
+                TileTerm tt;
+                std::memset(&tt, 0, sizeof(tt));
+                tt.x = it.x[t];
+                tt.z = it.z[t];
+                for (uint32_t i = 0; i < k; ++i)
+                    if ((tt.x >> tj.tpos[i]) & 1ull) tt.xl |= 1u << i;
+                tt.hl = tt.xl ? 31u - static_cast<uint32_t>(__builtin_clz(tt.xl)) : 0u;
+                tt.partial_off = partial_doubles;
+                ExpSlot sl;
+                std::memset(&sl, 0, sizeof(sl));
+                sl.partial_off = partial_doubles;
+                sl.blocks = tj.blocks;
+                sl.phase_k = __builtin_popcountll(tt.x & tt.z);
+                sl.out = it.out_index + t;
+                slots.push_back(sl);
+                partial_doubles += uint64_t(tj.blocks) * 2;
+                tterms.push_back(tt);
+            }
+            tblock_begin.push_back(tile_blocks);
+            tile_blocks += tj.blocks;
+            tjobs.push_back(tj);
+        }
+        for (uint64_t xm : wide) {
+            const std::vector<uint32_t>& members = groups[xm];
             for (std::size_t at = 0; at < members.size(); at += kTermsPerLaunch) {
                 const std::size_t cnt = std::min<std::size_t>(kTermsPerLaunch, members.size() - at);
                 ExpJob j;
                 std::memset(&j, 0, sizeof(j));
-                j.state_off = (addr - base_addr) / sizeof(V);
+                j.state_off = state_off;
                 j.x = xm;
-                j.h = xm ? 63u - static_cast<uint32_t>(__builtin_clzll(xm)) : 0u;
-                j.pairs = xm ? (uint64_t{1} << (it.n - 1)) : (uint64_t{1} << it.n);
+                j.h = 63u - static_cast<uint32_t>(__builtin_clzll(xm));
+                j.pairs = uint64_t{1} << (it.n - 1);
                 j.terms = static_cast<uint32_t>(cnt);
                 j.blocks = blocks_for(j.pairs);
                 j.partial_off = partial_doubles;
-                for (std::size_t k = 0; k < cnt; ++k) {
-                    const uint32_t t = members[at + k];
-                    j.z[k] = it.z[t];
-                    ExpSlot sl;
-                    std::memset(&sl, 0, sizeof(sl));
-                    sl.partial_off = partial_doubles + uint64_t(k) * j.blocks * 2;
-                    sl.blocks = j.blocks;
-                    sl.phase_k = __builtin_popcountll(xm & it.z[t]);
-                    sl.out = it.out_index + t;
-                    slots.push_back(sl);
-                }
-                partial_doubles += uint64_t(cnt) * j.blocks * 2;
+                for (std::size_t q = 0; q < cnt; ++q) j.z[q] = it.z[members[at + q]];
+                add_slots(xm, it.z, members, at, cnt, j.blocks, it.out_index);
                 block_begin.push_back(total_blocks);
                 total_blocks += j.blocks;
                 jobs.push_back(j);
             }
         }
     }
-    if (jobs.empty()) return;
+    if (slots.empty()) return;
     uint32_t out_terms = 0;
     for (const ExpSlot& sl : slots) out_terms = std::max(out_terms, sl.out + 1);
-    DeviceBuffer djobs(jobs.size() * sizeof(ExpJob)), dbegin(block_begin.size() * sizeof(uint32_t));
-    DeviceBuffer dslots(slots.size() * sizeof(ExpSlot)), dpartials(partial_doubles * sizeof(double));
-    DeviceBuffer dout(uint64_t(out_terms) * 2 * sizeof(double));
-    upload(djobs.get(), jobs.data(), jobs.size() * sizeof(ExpJob));
-    upload(dbegin.get(), block_begin.data(), block_begin.size() * sizeof(uint32_t));
+    DeviceBuffer dpartials(partial_doubles * sizeof(double));
+    DeviceBuffer dslots(slots.size() * sizeof(ExpSlot)), dout(uint64_t(out_terms) * 2 * sizeof(double));
     upload(dslots.get(), slots.data(), slots.size() * sizeof(ExpSlot));
-    hipLaunchKernelGGL(expectation_batch_kernel<V>, dim3(total_blocks), dim3(kThreads), 0, ctx.stream,
-                       static_cast<const ExpJob*>(djobs.as<ExpJob>()), static_cast<const uint32_t*>(dbegin.as<uint32_t>()),
-                       static_cast<uint32_t>(jobs.size()), reinterpret_cast<const V*>(base_addr), dpartials.as<double>());
-    MH_HIP(hipGetLastError());
+    DeviceBuffer dtjobs, dtgroups, dtbegin, djobs, dbegin;
+    if (!tjobs.empty()) {
+        dtjobs = DeviceBuffer(tjobs.size() * sizeof(TileJob));
+        dtgroups = DeviceBuffer(tterms.size() * sizeof(TileTerm));
+        dtbegin = DeviceBuffer(tblock_begin.size() * sizeof(uint32_t));
+        upload(dtjobs.get(), tjobs.data(), tjobs.size() * sizeof(TileJob));
+        upload(dtgroups.get(), tterms.data(), tterms.size() * sizeof(TileTerm));
+        upload(dtbegin.get(), tblock_begin.data(), tblock_begin.size() * sizeof(uint32_t));
+        hipLaunchKernelGGL(expectation_tile_kernel<V>, dim3(tile_blocks), dim3(kThreads), 0, ctx.stream,
+                           static_cast<const TileJob*>(dtjobs.as<TileJob>()),
+                           static_cast<const uint32_t*>(dtbegin.as<uint32_t>()), static_cast<uint32_t>(tjobs.size()),
+                           static_cast<const TileTerm*>(dtgroups.as<TileTerm>()), reinterpret_cast<const V*>(base_addr),
+                           dpartials.as<double>());
+        MH_HIP(hipGetLastError());
+    }
+    if (!jobs.empty()) {
+        djobs = DeviceBuffer(jobs.size() * sizeof(ExpJob));
+        dbegin = DeviceBuffer(block_begin.size() * sizeof(uint32_t));
+        upload(djobs.get(), jobs.data(), jobs.size() * sizeof(ExpJob));
+        upload(dbegin.get(), block_begin.data(), block_begin.size() * sizeof(uint32_t));
+        hipLaunchKernelGGL(expectation_batch_kernel<V>, dim3(total_blocks), dim3(kThreads), 0, ctx.stream,
+                           static_cast<const ExpJob*>(djobs.as<ExpJob>()), static_cast<const uint32_t*>(dbegin.as<uint32_t>()),
+                           static_cast<uint32_t>(jobs.size()), reinterpret_cast<const V*>(base_addr), dpartials.as<double>());
+        MH_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(expectation_finish_batch_kernel, dim3(static_cast<uint32_t>(slots.size())), dim3(kThreads), 0,
                        ctx.stream, static_cast<const ExpSlot*>(dslots.as<ExpSlot>()),
                        static_cast<const double*>(dpartials.as<double>()), dout.as<double>());
