@@ -193,8 +193,15 @@ struct TileTerm {
     uint64_t x, z;
     uint32_t xl;             // x on the tile bits
     uint32_t hl;             // msb of xl (x != 0)
+    uint32_t zl;             // z on the tile bits
+    uint32_t pad;
     uint64_t partial_off;    // partials[(block) * 2 + {0, 1}] from here
 };
+
+// d with its sign flipped when `neg`: a + flip(p, neg) is bitwise a - p or a + p.
+__device__ __forceinline__ double flip(double d, bool neg) {
+    return __longlong_as_double(__double_as_longlong(d) ^ (static_cast<long long>(neg) << 63));
+}
 
 template <typename V>
 __global__ void __launch_bounds__(kThreads)
@@ -246,38 +253,38 @@ __global__ void __launch_bounds__(kThreads)
         __syncthreads();  // the tables are written; the previous tile is no longer read
         for (uint32_t j = threadIdx.x; j < size; j += kThreads) tile[j] = s[tbase | dep_lo[j & 63u] | dep_hi[j >> 6]];
         __syncthreads();
+        // s(m) = (-1)^popcount(z & m) splits into the tile's outside bits (one
+        // parity per tile) and the tile bits (z and x taken onto them).
 #pragma unroll
         for (uint32_t t = 0; t < kTermsPerLaunch; ++t) {
             if (t >= nt) continue;
             const TileTerm& tt = terms[job.term_begin + t];
+            const uint32_t pz = static_cast<uint32_t>(__popcll(tt.z & tbase)) & 1u;
             if (tt.x == 0) {
                 for (uint32_t j = threadIdx.x; j < size; j += kThreads) {
-                    const uint64_t m = tbase | dep_lo[j & 63u] | dep_hi[j >> 6];
                     const double ar = static_cast<double>(tile[j].x), ai = static_cast<double>(tile[j].y);
                     const double pr = ar * ar + ai * ai;
                     const double pi = ar * ai - ai * ar;
-                    const bool neg = (__popcll(tt.z & m) & 1) != 0;
-                    accr[t] = neg ? accr[t] - pr : accr[t] + pr;
-                    acci[t] = neg ? acci[t] - pi : acci[t] + pi;
+                    const bool neg = ((__popc(tt.zl & j) ^ pz) & 1u) != 0;
+                    accr[t] = accr[t] + flip(pr, neg);
+                    acci[t] = acci[t] + flip(pi, neg);
                 }
             } else {
                 const uint32_t lowm = (1u << tt.hl) - 1u;
                 for (uint32_t r = threadIdx.x; r < size / 2; r += kThreads) {
                     const uint32_t j = ((r & ~lowm) << 1) | (r & lowm);
                     const uint32_t jp = j ^ tt.xl;
-                    const uint64_t m = tbase | dep_lo[j & 63u] | dep_hi[j >> 6];
-                    const uint64_t mp = m ^ tt.x;
                     const double ar = static_cast<double>(tile[j].x), ai = static_cast<double>(tile[j].y);
                     const double br = static_cast<double>(tile[jp].x), bi = static_cast<double>(tile[jp].y);
                     // p = conj(a) * b, with a at m (bit h of m clear) and b at m ^ x
                     const double pr = ar * br + ai * bi;
                     const double pi = ar * bi - ai * br;
-                    const bool neg_mp = (__popcll(tt.z & mp) & 1) != 0;
-                    const bool neg_m = (__popcll(tt.z & m) & 1) != 0;
-                    accr[t] = neg_mp ? accr[t] - pr : accr[t] + pr;
-                    acci[t] = neg_mp ? acci[t] - pi : acci[t] + pi;
-                    accr[t] = neg_m ? accr[t] - pr : accr[t] + pr;
-                    acci[t] = neg_m ? acci[t] + pi : acci[t] - pi;
+                    const bool neg_mp = ((__popc(tt.zl & jp) ^ pz) & 1u) != 0;  // s(m') for p
+                    const bool neg_m = ((__popc(tt.zl & j) ^ pz) & 1u) != 0;    // s(m) for conj(p)
+                    accr[t] = accr[t] + flip(pr, neg_mp);
+                    acci[t] = acci[t] + flip(pi, neg_mp);
+                    accr[t] = accr[t] + flip(pr, neg_m);
+                    acci[t] = acci[t] + flip(pi, !neg_m);
                 }
             }
         }
@@ -461,8 +468,10 @@ void expectation_batch_typed(const std::vector<ExpectationItem>& items, double* 
                 std::memset(&tt, 0, sizeof(tt));
                 tt.x = it.x[t];
                 tt.z = it.z[t];
-                for (uint32_t i = 0; i < k; ++i)
+                for (uint32_t i = 0; i < k; ++i) {
                     if ((tt.x >> tj.tpos[i]) & 1ull) tt.xl |= 1u << i;
+                    if ((tt.z >> tj.tpos[i]) & 1ull) tt.zl |= 1u << i;
+                }
                 tt.hl = tt.xl ? 31u - static_cast<uint32_t>(__builtin_clz(tt.xl)) : 0u;
                 tt.partial_off = partial_doubles;
                 ExpSlot sl;
