@@ -62,6 +62,19 @@ def test_batch_rounds_give_the_same_bits(hip):
     assert np.all(np.diff(many.completion_seconds) >= 0)
 
 
+def test_batch_groups_give_the_single_engine_bits(hip):
+    """A round of more than 64 circuits goes to the device in growing groups
+    (host planning of one group while the device runs the one before): every
+    circuit's terms equal the single engine's bit for bit."""
+    circuits, terms = W.batch_workload(90, 17, qubits=(10, 12), depth=(1, 4))
+    r = hip.batch_execute(circuits, terms)
+    assert r.rounds == 1
+    eng = hip.HipDenseEngine()
+    for i in range(0, 90, 9):
+        single = eng.execute(circuits[i], terms=terms[i], statevector=False, targets=[], width=0)
+        assert r.expectation[i].tobytes() == single.expectation.tobytes(), i
+
+
 def test_batch_complex64_error(hip):
     circuits, terms = W.batch_workload(48, 12, qubits=(11, 15), depth=(4, 12))
     ref = hip.batch_execute(circuits, terms, precision=hip.C128)

@@ -224,6 +224,29 @@ def test_expectation_terms(orc, hip):
             assert abs(g - want) <= 1e-12, (n, x, z, g, want)
 
 
+def test_expectation_tiles_and_wide_masks(orc, hip):
+    """Terms from LDS tiles (sets of up to 16 terms whose X bits fit an
+    11-qubit tile), sets split by their X bits and by their size, X masks too
+    wide for a tile (the per-mask kernel), identity and all-qubit strings, at
+    both precisions: within 1e-12 (complex64: 1e-5) of the sequential sum, and
+    the same bits alone and in a batch."""
+    rng = np.random.default_rng(77)
+    for n in (4, 11, 12, 15):
+        s = random_state(n, 60 + n)
+        terms = W.random_paulis(n, 40, n + 5, max_weight=min(4, n))
+        terms += W.random_paulis(n, 6, n + 6, max_weight=n, min_weight=min(n, 9))   # wide X masks
+        terms += [(0, 0), ((1 << n) - 1, 0), ((1 << n) - 1, (1 << n) - 1), (0, (1 << n) - 1)]
+        terms += [(int(x), int(rng.integers(0, 1 << n))) for x in rng.integers(0, 1 << n, 4)]
+        for prec, tol in ((hip.C128, 1e-12), (hip.C64, 1e-5)):
+            d = hip.DeviceState(n, prec)
+            d.upload(s.astype(np.complex64) if prec == hip.C64 else s)
+            got = d.expectation(terms)
+            d.free()
+            for (x, z), g in zip(terms, got):
+                want = orc.expectation_of_term(s, x, z)
+                assert abs(g - want) <= tol, (n, prec, x, z, g, want)
+
+
 def test_expectation_is_reproducible(hip):
     s = random_state(18, 5)
     terms = W.random_paulis(18, 8, 3)
