@@ -444,52 +444,71 @@ private:
         const double planned = now();
         ensure_arena(bytes);
         const double arena_ready = now();
-        uint64_t arena_at = 0;
         RoundStats rs;
-        for (const int32_t precision : {MH_C128, MH_C64}) {
-            std::vector<Job*> group;
-            for (std::size_t i = 0; i < round.size(); ++i)
-                if (statuses[i] == MH_OK && round[i]->precision == precision) group.push_back(round[i]);
-            if (group.empty()) continue;
-            std::vector<RoundItem> items(group.size());
-            uint32_t terms = 0;
-            uint64_t group_bytes = 0;
-            for (std::size_t k = 0; k < group.size(); ++k) {
-                Job& j = *group[k];
-                RoundItem& it = items[k];
-                it.n = j.circuit.view.num_qubits;
-                it.blocks = &j.blocks;
-                it.global_phase = j.global_phase;
-                it.num_terms = j.num_terms();
-                it.x = j.x.data();
-                it.z = j.z.data();
-                it.term_out = terms;
-                terms += j.num_terms();
-                it.shots = j.shots;
-                it.seed = j.seed;
-                it.targets = &j.targets;
-                it.counts = &j.counts;
-                group_bytes += round_state_bytes(it.n, precision);
+        // Parts: jobs whose state is at most an eighth of the round's largest
+        // run first and complete at once; the rest follow. A round that holds
+        // a large job then keeps its small jobs waiting for their own work only.
+        uint64_t largest = 0;
+        for (std::size_t i = 0; i < round.size(); ++i)
+            if (statuses[i] == MH_OK)
+                largest = std::max(largest, round_state_bytes(round[i]->circuit.view.num_qubits, round[i]->precision));
+        for (int part = 0; part < 2; ++part) {
+            uint64_t arena_at = 0;
+            std::vector<Job*> done_here;
+            for (const int32_t precision : {MH_C128, MH_C64}) {
+                std::vector<Job*> group;
+                for (std::size_t i = 0; i < round.size(); ++i) {
+                    if (statuses[i] != MH_OK || round[i]->precision != precision) continue;
+                    const bool small = round_state_bytes(round[i]->circuit.view.num_qubits, precision) * 8 <= largest;
+                    if (small == (part == 0)) group.push_back(round[i]);
+                }
+                if (group.empty()) continue;
+                std::vector<RoundItem> items(group.size());
+                uint32_t terms = 0;
+                uint64_t group_bytes = 0;
+                for (std::size_t k = 0; k < group.size(); ++k) {
+                    Job& j = *group[k];
+                    RoundItem& it = items[k];
+                    it.n = j.circuit.view.num_qubits;
+                    it.blocks = &j.blocks;
+                    it.global_phase = j.global_phase;
+                    it.num_terms = j.num_terms();
+                    it.x = j.x.data();
+                    it.z = j.z.data();
+                    it.term_out = terms;
+                    terms += j.num_terms();
+                    it.shots = j.shots;
+                    it.seed = j.seed;
+                    it.targets = &j.targets;
+                    it.counts = &j.counts;
+                    group_bytes += round_state_bytes(it.n, precision);
+                }
+                std::vector<double> raw(2 * static_cast<std::size_t>(terms));
+                try {
+                    execute_round(precision, items, threads_, raw.data(), &rs,
+                                  static_cast<unsigned char*>(arena_.get()) + arena_at);
+                } catch (const Error& e) {
+                    for (Job* j : group) mark_failed(*j, round, statuses, errors, e.status, e.what());
+                    continue;
+                } catch (const std::exception& e) {
+                    for (Job* j : group) mark_failed(*j, round, statuses, errors, MH_ERR_INTERNAL, e.what());
+                    continue;
+                }
+                arena_at += group_bytes;
+                for (std::size_t k = 0; k < group.size(); ++k) {
+                    Job& j = *group[k];
+                    const uint32_t at = items[k].term_out;
+                    j.raw_terms.assign(raw.begin() + 2 * at, raw.begin() + 2 * (at + j.num_terms()));
+                    if (j.kind == MH_JOB_ESTIMATOR) combine_observables(j);
+                    j.blocks.clear();
+                    j.blocks.shrink_to_fit();
+                    done_here.push_back(&j);
+                }
             }
-            std::vector<double> raw(2 * static_cast<std::size_t>(terms));
-            try {
-                execute_round(precision, items, threads_, raw.data(), &rs,
-                              static_cast<unsigned char*>(arena_.get()) + arena_at);
-            } catch (const Error& e) {
-                for (Job* j : group) mark_failed(*j, round, statuses, errors, e.status, e.what());
-                continue;
-            } catch (const std::exception& e) {
-                for (Job* j : group) mark_failed(*j, round, statuses, errors, MH_ERR_INTERNAL, e.what());
-                continue;
-            }
-            arena_at += group_bytes;
-            for (std::size_t k = 0; k < group.size(); ++k) {
-                Job& j = *group[k];
-                const uint32_t at = items[k].term_out;
-                j.raw_terms.assign(raw.begin() + 2 * at, raw.begin() + 2 * (at + j.num_terms()));
-                if (j.kind == MH_JOB_ESTIMATOR) combine_observables(j);
-                j.blocks.clear();
-                j.blocks.shrink_to_fit();
+            if (part == 0 && !done_here.empty()) {
+                std::lock_guard<std::mutex> lock(mu_);
+                for (Job* j : done_here) finish(j, MH_OK, "");
+                done_cv_.notify_all();
             }
         }
         std::lock_guard<std::mutex> lock(mu_);
