@@ -74,13 +74,16 @@ def batched(args, H, W) -> dict:
     `args.per_circuit_sample` circuits. Each form runs once untimed first."""
     import numpy as np
     circuits, terms = W.batch_workload(args.batch_circuits, 2026)
+    # complex64 (the tolerance form) keeps controlled-NOT blocks as controlled pairs; complex128 keeps the
+    # reference's containment plan. The per-circuit path uses the same fusion per precision.
+    fusion = {H.C64: H.FUSE_PAIRS, H.C128: H.FUSE_CONTAINMENT}
     for prec in (H.C64, H.C128):  # one untimed run of the whole batch per precision (steady state)
-        H.batch_execute(circuits, terms, precision=prec)
+        H.batch_execute(circuits, terms, precision=prec, fusion=fusion[prec])
     out = {"workload": f"{len(circuits)} ansatz circuits, n 10..22, depth 4..12, 8 Paulis each",
-           "circuits": len(circuits)}
+           "circuits": len(circuits), "fusion": {"c64": "pairs", "c128": "containment"}}
     res = {}
     for name, prec in (("c64", H.C64), ("c128", H.C128)):
-        r = H.batch_execute(circuits, terms, precision=prec)
+        r = H.batch_execute(circuits, terms, precision=prec, fusion=fusion[prec])
         res[name] = r
         lat = np.sort(r.completion_seconds) * 1e3
         out[name] = {"circuits_per_s": round(len(circuits) / r.total_seconds, 1),
@@ -93,12 +96,13 @@ def batched(args, H, W) -> dict:
                                                for a, b in zip(res["c64"].expectation, res["c128"].expectation)))
     k = min(args.per_circuit_sample, len(circuits))
     eng = H.HipDenseEngine()
-    eng.execute(circuits[0], precision=H.C64, terms=terms[0], statevector=False, targets=[], width=0)
+    eng.execute(circuits[0], precision=H.C64, fusion=fusion[H.C64], terms=terms[0], statevector=False, targets=[],
+                width=0)
     per = {}
     for name, prec in (("c64", H.C64), ("c128", H.C128)):
         t0 = time.perf_counter()
         for c, t in zip(circuits[:k], terms[:k]):
-            eng.execute(c, precision=prec, terms=t, statevector=False, targets=[], width=0)
+            eng.execute(c, precision=prec, fusion=fusion[prec], terms=t, statevector=False, targets=[], width=0)
         per[name] = k / (time.perf_counter() - t0)
     out["per_circuit_path"] = {"sample": k, "c64_circuits_per_s": round(per["c64"], 1),
                                "c128_circuits_per_s": round(per["c128"], 1)}

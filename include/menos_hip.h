@@ -161,10 +161,16 @@ typedef struct mh_circuit {
 
 /* How the gate list becomes blocks. MH_FUSE_NONE: one block per gate.
  * MH_FUSE_CONTAINMENT: the reference's rule (src/ir/fuse_gates.cpp:190-283),
- * its default. MH_FUSE_WIDTH: this engine's width-capped fusion (fusion_width
+ * its default. MH_FUSE_PAIRS: the containment rule, except that a gate does not
+ * join a block containing a CX if it would act non-diagonally on that CX's
+ * control (nor a CX whose control such a gate touches), so those blocks stay
+ * controlled pairs (a 2x2 block on the target per control value), which cost
+ * half of a dense 2-qubit block; no reference counterpart (the reference
+ * applies every fused block the same way). MH_FUSE_WIDTH: this engine's
+ * width-capped fusion (fusion_width
  * operands at most, 2..6), order-preserving; the oracle applies the same
  * block list. */
-typedef enum mh_fusion { MH_FUSE_NONE = 0, MH_FUSE_CONTAINMENT = 1, MH_FUSE_WIDTH = 2 } mh_fusion;
+typedef enum mh_fusion { MH_FUSE_NONE = 0, MH_FUSE_CONTAINMENT = 1, MH_FUSE_WIDTH = 2, MH_FUSE_PAIRS = 3 } mh_fusion;
 
 typedef struct mh_execution_request {
     const mh_circuit* circuit;

@@ -192,12 +192,15 @@ FusionPlan finish_plan(const std::vector<uint32_t>& block_of, const std::vector<
 // 190-283): a gate joins the newest open block sharing a qubit with it only if
 // one qubit set contains the other (diagonal rule aside); a measurement is a
 // barrier block that closes every open block; blocks run in opening order.
-FusionPlan containment_plan(const CircuitView& cv) {
+FusionPlan containment_plan(const CircuitView& cv, bool keep_pairs) {
     constexpr uint32_t kNone = std::numeric_limits<uint32_t>::max();
     const uint32_t count = cv.num_ops();
     std::vector<uint32_t> block_of(count);
     std::vector<uint64_t> mask;
     std::vector<uint8_t> diag;
+    // keep_pairs: per block, the control qubit of its controlled NOTs and the
+    // qubits its non-diagonal gates other than those act on.
+    std::vector<uint64_t> ctrl, dense;
     std::array<uint32_t, 64> newest{};
     newest.fill(kNone);
     for (uint32_t i = 0; i < count; ++i) {
@@ -205,6 +208,8 @@ FusionPlan containment_plan(const CircuitView& cv) {
             block_of[i] = static_cast<uint32_t>(mask.size());
             mask.push_back(0);
             diag.push_back(0);
+            ctrl.push_back(0);
+            dense.push_back(0);
             newest.fill(kNone);
             continue;
         }
@@ -215,15 +220,34 @@ FusionPlan containment_plan(const CircuitView& cv) {
             const uint32_t b = newest[static_cast<std::size_t>(std::countr_zero(r))];
             if (b != kNone && (dep == kNone || b > dep)) dep = b;
         }
+        const bool cnot = cv.opcode(i) == kCX;
+        const uint64_t gc = cnot ? uint64_t{1} << cv.qubits(i)[0] : 0;   // the control
+        const uint64_t gdense = gd ? 0 : (cnot ? gm & ~gc : gm);
+        // A block that stays block-diagonal in one control qubit applies as a
+        // controlled pair: no non-diagonal gate may touch the control.
+        auto keeps_pair = [&](uint32_t b) {
+            const uint64_t c = ctrl[b] | gc;
+            return std::popcount(c) <= 1 && ((dense[b] | gdense) & c) == 0;
+        };
         uint32_t blk;
-        if (dep != kNone && may_join(mask[dep], diag[dep] != 0, gm, gd)) {
+        if (dep != kNone && may_join(mask[dep], diag[dep] != 0, gm, gd) &&
+            (!keep_pairs || (ctrl[dep] == 0 && gc == 0) || keeps_pair(dep))) {
             blk = dep;
             mask[blk] |= gm;
             diag[blk] = static_cast<uint8_t>(diag[blk] != 0 && gd);
+            if (keep_pairs && keeps_pair(blk)) {
+                ctrl[blk] |= gc;
+                dense[blk] |= gdense;
+            } else {
+                ctrl[blk] = 0;
+                dense[blk] = ~uint64_t{0};
+            }
         } else {
             blk = static_cast<uint32_t>(mask.size());
             mask.push_back(gm);
             diag.push_back(gd ? 1 : 0);
+            ctrl.push_back(gc);
+            dense.push_back(gdense);
         }
         block_of[i] = blk;
         for (uint64_t r = gm; r != 0; r &= r - 1) newest[static_cast<std::size_t>(std::countr_zero(r))] = blk;
@@ -275,7 +299,8 @@ FusionPlan unfused_plan(const CircuitView& cv) {
 FusionPlan plan_fusion(const CircuitView& cv, int32_t fusion, uint32_t width) {
     switch (fusion) {
         case MH_FUSE_NONE: return unfused_plan(cv);
-        case MH_FUSE_CONTAINMENT: return containment_plan(cv);
+        case MH_FUSE_CONTAINMENT: return containment_plan(cv, false);
+        case MH_FUSE_PAIRS: return containment_plan(cv, true);
         case MH_FUSE_WIDTH:
             if (width < 1 || width > 6) fail(MH_ERR_INVALID, "fusion_width must be 1..6");
             return width_plan(cv, width);

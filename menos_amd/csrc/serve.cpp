@@ -580,7 +580,8 @@ int mh_server_create(const mh_server_options* options, mh_server** out) {
         mh_server_options o{};
         if (options) o = *options;
         if (options == nullptr) o.fusion = MH_FUSE_CONTAINMENT;
-        if (o.fusion != MH_FUSE_NONE && o.fusion != MH_FUSE_CONTAINMENT && o.fusion != MH_FUSE_WIDTH)
+        if (o.fusion != MH_FUSE_NONE && o.fusion != MH_FUSE_CONTAINMENT && o.fusion != MH_FUSE_WIDTH &&
+            o.fusion != MH_FUSE_PAIRS)
             fail(MH_ERR_INVALID, "unknown fusion mode");
         context();
         auto s = std::make_unique<mh_server>();

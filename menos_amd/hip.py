@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmenos_hip.so")
 
 C128, C64 = 0, 1
-FUSE_NONE, FUSE_CONTAINMENT, FUSE_WIDTH = 0, 1, 2
+FUSE_NONE, FUSE_CONTAINMENT, FUSE_WIDTH, FUSE_PAIRS = 0, 1, 2, 3
 MH_OK, MH_ERR_INVALID, MH_ERR_RESOURCE, MH_ERR_INTERNAL, MH_ERR_DEVICE = range(5)
 
 # Every entry point include/menos_hip.h declares.

@@ -77,6 +77,31 @@ def test_width_fusion_matches_oracle_on_same_plan(orc, hip, width):
     assert np.max(np.abs(r.expectation - e)) <= 1e-12
 
 
+@pytest.mark.parametrize("kind", ["ansatz", "layered", "random"])
+def test_pair_fusion_matches_oracle_on_same_plan(orc, hip, kind):
+    """Containment fusion that keeps controlled-NOT blocks as controlled pairs:
+    complex128 bit for bit the oracle applying the same block list, complex64
+    within 1e-5; an ansatz fuses into one 1-qubit block and n - 1 pairs per layer."""
+    n = 14
+    if kind == "ansatz":
+        c = W.ansatz_circuit(n, 4, np.random.default_rng(3).uniform(-np.pi, np.pi, 2 * n * 4).tolist())
+    elif kind == "layered":
+        c = W.layered(n, 5)
+    else:
+        c = W.random_circuit(n, 4, 9)
+    order, bb, bq, qb, _, _ = hip.plan_fusion(c, hip.FUSE_PAIRS)
+    plan = oracle.Plan(order, bb, bq, qb)
+    terms = W.random_paulis(n, 12, 4)
+    want, _, e = orc.execute(c, plan=plan, terms=terms)
+    r = hip.HipDenseEngine().execute(c, fusion=hip.FUSE_PAIRS, terms=terms, targets=[], width=0)
+    assert r.statevector.tobytes() == want.tobytes()
+    assert np.max(np.abs(r.expectation - e)) <= 1e-12
+    r64 = hip.HipDenseEngine().execute(c, precision=hip.C64, fusion=hip.FUSE_PAIRS, terms=terms, targets=[], width=0)
+    assert np.max(np.abs(r64.statevector.astype(np.complex128) - want)) <= 1e-5
+    if kind == "ansatz":
+        assert plan.num_blocks == 4 * n
+
+
 @pytest.mark.skip(reason=WIDE_WINDOW_SKIP)
 @pytest.mark.parametrize("seed", range(4))
 def test_mixed_circuits(orc, hip, seed):
