@@ -137,11 +137,12 @@ def served(args, H, W) -> dict:
     log(f"served: {len(jobs)} jobs built in {time.perf_counter() - t0:.1f} s")
     out = {"workload": f"{len(jobs)} jobs, n log-uniform 12..30; 70% estimator (ansatz depth 4..12, 16 observables, "
                        f"{args.serve_estimator_precision}), 30% sampler (U3 + CZ brick, 20 layers, 1000 shots, "
-                       "c128); 40% parameter-only updates; 256 GB ledger budget",
+                       "c128); 40% parameter-only updates; 256 GB ledger budget; complex64 jobs with pair-keeping "
+                       "fusion",
            "jobs": len(jobs),
            "estimators": sum(s.kind == "estimator" for s in specs),
            "updates": sum(s.update for s in specs)}
-    with H.Server(budget_bytes=256_000_000_000, reserve_bytes=256_000_000_000) as srv:
+    with H.Server(budget_bytes=256_000_000_000, reserve_bytes=256_000_000_000, complex64_pairs=True) as srv:
         w0 = time.perf_counter()
         res = srv.replay(jobs, np.zeros(len(jobs)))
         wall = time.perf_counter() - w0
@@ -162,7 +163,7 @@ def served(args, H, W) -> dict:
     rate = args.serve_rate or 0.8 * cap
     arrivals = gaps / rate
     jobs = jobs_of()
-    with H.Server(budget_bytes=256_000_000_000, reserve_bytes=256_000_000_000) as srv:
+    with H.Server(budget_bytes=256_000_000_000, reserve_bytes=256_000_000_000, complex64_pairs=True) as srv:
         w0 = time.perf_counter()
         res = srv.replay(jobs, arrivals)
         wall = time.perf_counter() - w0

@@ -331,6 +331,7 @@ typedef struct mh_server_options {
     uint32_t plan_cache_capacity;   /* cached fusion plans; 0: 256 */
     uint64_t reserve_bytes;         /* device bytes for round states allocated at creation (at most the
                                        budget); 0: the arena grows with the rounds */
+    int32_t complex64_pairs;        /* nonzero: complex64 jobs use MH_FUSE_PAIRS (complex128 jobs keep `fusion`) */
 } mh_server_options;
 
 typedef enum mh_job_kind { MH_JOB_ESTIMATOR = 0, MH_JOB_SAMPLER = 1 } mh_job_kind;

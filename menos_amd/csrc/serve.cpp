@@ -425,11 +425,13 @@ private:
             j.started = started;
             try {
                 const CircuitView cv{&j.circuit.view};
-                const std::string key = structure_key(&j.circuit.view, options_.fusion, options_.fusion_width);
+                const int32_t fusion =
+                    j.precision == MH_C64 && options_.complex64_pairs ? int32_t{MH_FUSE_PAIRS} : options_.fusion;
+                const std::string key = structure_key(&j.circuit.view, fusion, options_.fusion_width);
                 std::shared_ptr<const FusionPlan> fp = cache_.find(key);
                 j.cache_hit = fp != nullptr;
                 if (!fp) {
-                    fp = std::make_shared<const FusionPlan>(plan_fusion(cv, options_.fusion, options_.fusion_width));
+                    fp = std::make_shared<const FusionPlan>(plan_fusion(cv, fusion, options_.fusion_width));
                     cache_.insert(key, fp);
                 }
                 j.blocks = build_blocks(cv, *fp);

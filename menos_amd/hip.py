@@ -380,7 +380,8 @@ def batch_execute(circuits, terms, *, precision=C128, fusion=FUSE_CONTAINMENT, f
 class CServerOptions(ctypes.Structure):
     _fields_ = [("budget_bytes", ctypes.c_uint64), ("max_round_jobs", ctypes.c_uint32),
                 ("host_threads", ctypes.c_uint32), ("fusion", ctypes.c_int32), ("fusion_width", ctypes.c_uint32),
-                ("plan_cache_capacity", ctypes.c_uint32), ("reserve_bytes", ctypes.c_uint64)]
+                ("plan_cache_capacity", ctypes.c_uint32), ("reserve_bytes", ctypes.c_uint64),
+                ("complex64_pairs", ctypes.c_int32)]
 
 
 class CJob(ctypes.Structure):
@@ -477,9 +478,9 @@ class Server:
     """A resident job server (mh_server_*): submit jobs, wait for results."""
 
     def __init__(self, *, budget_bytes=0, max_round_jobs=0, host_threads=0, fusion=FUSE_CONTAINMENT,
-                 fusion_width=0, plan_cache_capacity=0, reserve_bytes=0):
+                 fusion_width=0, plan_cache_capacity=0, reserve_bytes=0, complex64_pairs=False):
         o = CServerOptions(budget_bytes, max_round_jobs, host_threads, fusion, fusion_width, plan_cache_capacity,
-                           reserve_bytes)
+                           reserve_bytes, int(bool(complex64_pairs)))
         self._h = ctypes.c_void_p()
         check(lib().mh_server_create(ctypes.byref(o), ctypes.byref(self._h)))
         self._pending = {}

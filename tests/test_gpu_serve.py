@@ -133,14 +133,19 @@ def test_replay_gives_the_closed_loop_results(hip):
             assert a.counts == b.counts
 
 
-def test_complex64_estimator_error(hip):
+@pytest.mark.parametrize("pairs", [False, True])
+def test_complex64_estimator_error(hip, pairs):
+    """complex64 estimator jobs (under the containment plan or, with
+    complex64_pairs, the pair-keeping one) within 1e-5 of complex128 jobs, whose
+    raw terms stay the single engine's under the containment plan."""
     specs = [_estimator(hip, 12 + i, 4, 200 + i, precision=1) for i in range(3)]
-    with hip.Server() as s:
+    with hip.Server(complex64_pairs=pairs) as s:
         r64 = [s.wait(s.submit(j)) for _, _, j in specs]
         r128 = [s.wait(s.submit(hip.Job(hip.JOB_ESTIMATOR, c, observables=obs))) for c, obs, _ in specs]
-    for a, b in zip(r64, r128):
+    for (c, obs, _), a, b in zip(specs, r64, r128):
         assert a.status == 0 and b.status == 0
         assert np.max(np.abs(a.values - b.values)) <= 1e-5
+        _check_estimator(hip, c, obs, b)
 
 
 def test_invalid_jobs_are_refused_at_submission(hip):
