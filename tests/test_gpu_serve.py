@@ -157,3 +157,24 @@ def test_invalid_jobs_are_refused_at_submission(hip):
         with pytest.raises(hip.HipError):
             s.submit(hip.Job(hip.JOB_SAMPLER, c12, precision=hip.C64, shots=10))
         assert s.stats().jobs_completed == 0
+
+
+def test_ticket_rules_and_shutdown(hip):
+    """A ticket is waited for once; unknown tickets and decreasing arrival
+    times are refused; a server closes with jobs nobody waited for."""
+    c, obs, j = _estimator(hip, 11, 2, 300)
+    s = hip.Server()
+    t = s.submit(j)
+    r = s.wait(t)
+    assert r.status == 0
+    s._pending[t] = j                       # ask again for the collected ticket
+    with pytest.raises(hip.HipError):
+        s.wait(t)
+    s._pending[12345] = j
+    with pytest.raises(hip.HipError):
+        s.wait(12345)
+    with pytest.raises(hip.HipError):
+        s.replay([j, j], [0.2, 0.1])
+    for _ in range(3):
+        s.submit(j)                          # left for the shutdown to complete
+    s.close()
