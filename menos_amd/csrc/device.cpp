@@ -46,6 +46,7 @@ Context& context() {
             fail(MH_ERR_DEVICE, std::string("device is ") + c->props.gcnArchName + ", this library is built for gfx950");
         }
         MH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        MH_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
         MH_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
         return c;
     }();

@@ -45,11 +45,12 @@ int guarded(F&& f) noexcept {
     }
 }
 
-// The library's device context: two streams of default priority on device 0,
+// The library's device context: three streams of default priority on device 0,
 // created on first use.
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;       // kernels
+    hipStream_t stream2 = nullptr;      // a second kernel stream: batched groups alternate between the two
     hipStream_t copy_stream = nullptr;  // table uploads that run ahead of the kernels
     hipDeviceProp_t props{};
 };

@@ -1191,7 +1191,7 @@ void keep_upload(BatchKeep& keep, DeviceBuffer& dst, std::vector<T>&& src, hipSt
 // the compute stream, and `keep` holds what they read.
 template <typename R>
 void launch_window_passes_batch(const BatchItem* items, std::size_t count, uint32_t threads, BatchStats* stats,
-                                BatchKeep& keep) {
+                                BatchKeep& keep, hipStream_t stream) {
     using V = typename Shape<R>::V;
     constexpr uint32_t W = width_of<R>();
     Context& ctx = context();
@@ -1330,7 +1330,13 @@ void launch_window_passes_batch(const BatchItem* items, std::size_t count, uint3
     keep_upload(keep, dgroups, std::move(all_groups), ctx.copy_stream);
     MH_HIP(hipEventCreateWithFlags(&keep.copied, hipEventDisableTiming));
     MH_HIP(hipEventRecord(keep.copied, ctx.copy_stream));
-    MH_HIP(hipStreamWaitEvent(ctx.stream, keep.copied, 0));
+    // A generic pass runs on the main stream (launch_generic), so a group that
+    // has one stays there in full.
+    bool has_generic = false;
+    for (std::size_t i = 0; i < count && !has_generic; ++i)
+        for (const PassRec& rec : comp[i].passes) has_generic = has_generic || rec.generic;
+    if (has_generic) stream = ctx.stream;
+    MH_HIP(hipStreamWaitEvent(stream, keep.copied, 0));
     keep.device.push_back(std::move(dbundles));
     keep.device.push_back(std::move(dops));
     keep.device.push_back(std::move(dterms));
@@ -1344,11 +1350,11 @@ void launch_window_passes_batch(const BatchItem* items, std::size_t count, uint3
         if (list_len[p] > 0) {
             if (pairs_in[p])
                 hipLaunchKernelGGL((window_batch_kernel<R, true>), dim3(total_groups[p]), dim3(kWavesPerGroup * 64), 0,
-                                   ctx.stream, dargs_p + args_at[p], dgroups_p + groups_at[p],
+                                   stream, dargs_p + args_at[p], dgroups_p + groups_at[p],
                                    static_cast<uint32_t>(list_len[p]), state_base, dbundles_p, dops_p, dterms_p, dmats_p);
             else
                 hipLaunchKernelGGL((window_batch_kernel<R, false>), dim3(total_groups[p]), dim3(kWavesPerGroup * 64), 0,
-                                   ctx.stream, dargs_p + args_at[p], dgroups_p + groups_at[p],
+                                   stream, dargs_p + args_at[p], dgroups_p + groups_at[p],
                                    static_cast<uint32_t>(list_len[p]), state_base, dbundles_p, dops_p, dterms_p, dmats_p);
             MH_HIP(hipGetLastError());
             ++launches;
@@ -1430,6 +1436,8 @@ std::vector<WindowPass> plan_window_passes(uint32_t n, int32_t precision, const 
 struct BatchPassRunner::Impl {
     int32_t precision = MH_C128;
     uint32_t threads = 1;
+    uint32_t added = 0;
+    hipEvent_t start = nullptr;  // the work queued before the runner, for the second stream to wait on
     std::vector<BatchKeep> keeps;
 };
 
@@ -1437,16 +1445,22 @@ BatchPassRunner::BatchPassRunner(int32_t precision, uint32_t threads) : impl_(st
     amp_bytes(precision);
     impl_->precision = precision;
     impl_->threads = threads;
+    Context& ctx = context();
+    MH_HIP(hipEventCreateWithFlags(&impl_->start, hipEventDisableTiming));
+    MH_HIP(hipEventRecord(impl_->start, ctx.stream));
+    MH_HIP(hipStreamWaitEvent(ctx.stream2, impl_->start, 0));
 }
 
 BatchPassRunner::~BatchPassRunner() {
     // Whatever was launched or copied reads the kept tables: wait before they go.
     if (!impl_->keeps.empty()) {
         (void)hipStreamSynchronize(context().copy_stream);
+        (void)hipStreamSynchronize(context().stream2);
         (void)hipStreamSynchronize(context().stream);
     }
     for (BatchKeep& k : impl_->keeps)
         if (k.copied) (void)hipEventDestroy(k.copied);
+    if (impl_->start) (void)hipEventDestroy(impl_->start);
 }
 
 void BatchPassRunner::add(const BatchItem* items, std::size_t count, BatchStats* stats) {
@@ -1455,14 +1469,19 @@ void BatchPassRunner::add(const BatchItem* items, std::size_t count, BatchStats*
     for (std::size_t i = 0; i < count; ++i)
         if (items[i].n < width) fail(MH_ERR_INVALID, "a batched state has fewer qubits than one window");
     impl_->keeps.emplace_back();
+    // Groups alternate between the two kernel streams, so the short launches at
+    // the end of one group overlap the next group's.
+    Context& ctx = context();
+    const hipStream_t stream = (impl_->added++ % 2 == 0) ? ctx.stream : ctx.stream2;
     if (impl_->precision == MH_C128)
-        launch_window_passes_batch<double>(items, count, impl_->threads, stats, impl_->keeps.back());
+        launch_window_passes_batch<double>(items, count, impl_->threads, stats, impl_->keeps.back(), stream);
     else
-        launch_window_passes_batch<float>(items, count, impl_->threads, stats, impl_->keeps.back());
+        launch_window_passes_batch<float>(items, count, impl_->threads, stats, impl_->keeps.back(), stream);
 }
 
 void BatchPassRunner::finish() {
     MH_HIP(hipStreamSynchronize(context().copy_stream));
+    MH_HIP(hipStreamSynchronize(context().stream2));
     synchronize();
     for (BatchKeep& k : impl_->keeps)
         if (k.copied) (void)hipEventDestroy(k.copied);
