@@ -203,8 +203,67 @@ __device__ __forceinline__ double flip(double d, bool neg) {
     return __longlong_as_double(__double_as_longlong(d) ^ (static_cast<long long>(neg) << 63));
 }
 
+// One term over one tile in LDS, for thread u. Thread u handles tile entries
+// u, u + kThreads, ... (pairs r = u, u + kThreads, ... for an X term). Entry
+// and pair indices split into the thread's bits and a step that is the same
+// across the workgroup, and both the index map and the sign parity are linear
+// over that split: the per-step parts are uniform, so each entry costs two
+// index XORs and one sign XOR. s(m ^ x) = s(m) ^ popcount(z & x). kFull: the
+// tile has 2^kTileBits entries, so no entry is out of range.
+template <bool kFull, typename V>
+__device__ __forceinline__ void tile_term(const V* tile, const TileTerm& tt, uint32_t pz, uint32_t u, uint32_t size,
+                                          double& accr, double& acci) {
+    if (tt.x == 0) {
+        const uint32_t su = (static_cast<uint32_t>(__popc(tt.zl & u)) ^ pz) & 1u;
+#pragma unroll
+        for (uint32_t step = 0; step < (1u << kTileBits); step += kThreads) {
+            const uint32_t j = step | u;
+            if (!kFull && j >= size) continue;
+            const double ar = static_cast<double>(tile[j].x), ai = static_cast<double>(tile[j].y);
+            const double pr = ar * ar + ai * ai;
+            const double pi = ar * ai - ai * ar;
+            const bool neg = ((su ^ static_cast<uint32_t>(__popc(tt.zl & step))) & 1u) != 0;
+            accr = accr + flip(pr, neg);
+            acci = acci + flip(pi, neg);
+        }
+    } else {
+        // Pair r -> m: r's bits with a zero inserted at bit hl (the top bit of xl).
+        const uint32_t lowm = (1u << tt.hl) - 1u;
+        const uint32_t ju = ((u & ~lowm) << 1) | (u & lowm);
+        const uint32_t jpu = ju ^ tt.xl;
+        const uint32_t su = (static_cast<uint32_t>(__popc(tt.zl & ju)) ^ pz) & 1u;
+        const uint32_t sx = static_cast<uint32_t>(__popc(tt.zl & tt.xl)) & 1u;
+#pragma unroll
+        for (uint32_t step = 0; step < (1u << kTileBits) / 2; step += kThreads) {
+            if (!kFull && (step | u) >= size / 2) continue;
+            const uint32_t js = ((step & ~lowm) << 1) | (step & lowm);
+            const uint32_t j = ju | js, jp = jpu ^ js;
+            const double ar = static_cast<double>(tile[j].x), ai = static_cast<double>(tile[j].y);
+            const double br = static_cast<double>(tile[jp].x), bi = static_cast<double>(tile[jp].y);
+            // p = conj(a) * b, with a at m (bit h of m clear) and b at m ^ x
+            const double pr = ar * br + ai * bi;
+            const double pi = ar * bi - ai * br;
+            const uint32_t sm = (su ^ static_cast<uint32_t>(__popc(tt.zl & js))) & 1u;
+            const bool neg_m = sm != 0;           // s(m) for conj(p)
+            const bool neg_mp = (sm ^ sx) != 0;   // s(m') for p
+            accr = accr + flip(pr, neg_mp);
+            acci = acci + flip(pi, neg_mp);
+            accr = accr + flip(pr, neg_m);
+            acci = acci + flip(pi, !neg_m);
+        }
+    }
+}
+
+// Five waves per SIMD: the per-thread sums (16 terms, real and imaginary, in
+// double) would otherwise leave the kernel at 161 VGPRs (three waves), and the kernel waits on LDS and
+// memory latency. complex128 reaches four (its 32 KB tile caps the CU at four
+// workgroups) but still gets the smaller register budget, without spills,
+// which the four-wave bound does not give it; the compiler's note that five
+// is out of reach there is silenced.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wpass-failed"
 template <typename V>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads, 5)
     expectation_tile_kernel(const TileJob* __restrict__ jobs, const uint32_t* __restrict__ block_begin,
                             uint32_t count, const TileTerm* __restrict__ terms, const V* base, double* partials) {
     __shared__ V tile[1u << kTileBits];
@@ -255,38 +314,18 @@ __global__ void __launch_bounds__(kThreads)
         __syncthreads();
         // s(m) = (-1)^popcount(z & m) splits into the tile's outside bits (one
         // parity per tile) and the tile bits (z and x taken onto them).
+        // Whole tiles take the form without range checks, whose LDS reads
+        // the compiler can issue together; both forms add in the same order.
+        const uint32_t u = threadIdx.x;
 #pragma unroll
         for (uint32_t t = 0; t < kTermsPerLaunch; ++t) {
             if (t >= nt) continue;
             const TileTerm& tt = terms[job.term_begin + t];
             const uint32_t pz = static_cast<uint32_t>(__popcll(tt.z & tbase)) & 1u;
-            if (tt.x == 0) {
-                for (uint32_t j = threadIdx.x; j < size; j += kThreads) {
-                    const double ar = static_cast<double>(tile[j].x), ai = static_cast<double>(tile[j].y);
-                    const double pr = ar * ar + ai * ai;
-                    const double pi = ar * ai - ai * ar;
-                    const bool neg = ((__popc(tt.zl & j) ^ pz) & 1u) != 0;
-                    accr[t] = accr[t] + flip(pr, neg);
-                    acci[t] = acci[t] + flip(pi, neg);
-                }
-            } else {
-                const uint32_t lowm = (1u << tt.hl) - 1u;
-                for (uint32_t r = threadIdx.x; r < size / 2; r += kThreads) {
-                    const uint32_t j = ((r & ~lowm) << 1) | (r & lowm);
-                    const uint32_t jp = j ^ tt.xl;
-                    const double ar = static_cast<double>(tile[j].x), ai = static_cast<double>(tile[j].y);
-                    const double br = static_cast<double>(tile[jp].x), bi = static_cast<double>(tile[jp].y);
-                    // p = conj(a) * b, with a at m (bit h of m clear) and b at m ^ x
-                    const double pr = ar * br + ai * bi;
-                    const double pi = ar * bi - ai * br;
-                    const bool neg_mp = ((__popc(tt.zl & jp) ^ pz) & 1u) != 0;  // s(m') for p
-                    const bool neg_m = ((__popc(tt.zl & j) ^ pz) & 1u) != 0;    // s(m) for conj(p)
-                    accr[t] = accr[t] + flip(pr, neg_mp);
-                    acci[t] = acci[t] + flip(pi, neg_mp);
-                    accr[t] = accr[t] + flip(pr, neg_m);
-                    acci[t] = acci[t] + flip(pi, !neg_m);
-                }
-            }
+            if (size == (1u << kTileBits))
+                tile_term<true>(tile, tt, pz, u, size, accr[t], acci[t]);
+            else
+                tile_term<false>(tile, tt, pz, u, size, accr[t], acci[t]);
         }
     }
     // Fixed-order tree over the workgroup, one term at a time.
@@ -316,6 +355,8 @@ __global__ void __launch_bounds__(kThreads)
         __syncthreads();
     }
 }
+
+#pragma clang diagnostic pop
 
 // One term slot of a batched launch.
 struct ExpSlot {
