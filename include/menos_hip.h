@@ -23,9 +23,9 @@
  * stores complex64 amplitudes, computes gates in FP32 and keeps norms,
  * expectations and probability prefix sums in FP64.
  *
- * Every call that touches device memory synchronises its stream before it
- * returns (reference dense_kernels.h:204-206), so a following call sees its
- * writes. Device pointers passed in are never freed by the callee.
+ * Every call that touches device memory synchronises the streams it used
+ * before it returns (reference dense_kernels.h:204-206), so a following call
+ * sees its writes. Device pointers passed in are never freed by the callee.
  */
 #ifndef MENOS_HIP_H
 #define MENOS_HIP_H
