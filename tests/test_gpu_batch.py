@@ -62,17 +62,20 @@ def test_batch_rounds_give_the_same_bits(hip):
     assert np.all(np.diff(many.completion_seconds) >= 0)
 
 
-def test_batch_groups_give_the_single_engine_bits(hip):
+def test_batch_groups_give_the_single_engine_bits(orc, hip):
     """A round of more than 64 circuits goes to the device in growing groups
-    (host planning of one group while the device runs the one before): every
-    circuit's terms equal the single engine's bit for bit."""
+    (host planning of one group while the device runs the one before), which
+    alternate between two kernel streams: every circuit's terms equal the
+    single engine's bit for bit, in every group."""
     circuits, terms = W.batch_workload(90, 17, qubits=(10, 12), depth=(1, 4))
-    r = hip.batch_execute(circuits, terms)
+    r = hip.batch_execute(circuits, terms, statevectors=True)
     assert r.rounds == 1
     eng = hip.HipDenseEngine()
-    for i in range(0, 90, 9):
+    for i in range(90):
         single = eng.execute(circuits[i], terms=terms[i], statevector=False, targets=[], width=0)
         assert r.expectation[i].tobytes() == single.expectation.tobytes(), i
+        want, _, _ = orc.execute(circuits[i], fuse_gates=True)
+        assert r.statevectors[i].tobytes() == want.tobytes(), i
 
 
 def test_batch_complex64_error(hip):
