@@ -54,8 +54,11 @@ struct BatchStats {
 // Applies the blocks of groups of items as they are added, without waiting:
 // add() plans and compiles a group's passes on host threads, copies their
 // tables on the copy stream and launches them behind the work already queued
-// (so the host prepares the next group while the device runs this one);
-// finish() synchronises. The destructor waits for what was launched.
+// (so the host prepares the next group while the device runs this one).
+// Groups alternate between the two kernel streams, both of which start behind
+// the work queued on the main stream before the runner was made; the groups
+// must touch disjoint states. finish() synchronises. The destructor waits for
+// what was launched.
 class BatchPassRunner {
 public:
     BatchPassRunner(int32_t precision, uint32_t threads);

@@ -1188,7 +1188,8 @@ void keep_upload(BatchKeep& keep, DeviceBuffer& dst, std::vector<T>&& src, hipSt
 // one launch of window_batch_kernel (blocks the window kernel cannot apply run
 // on the generic kernel, state by state, after that launch). Returns without
 // waiting: the tables go up on the copy stream, the launches wait for them on
-// the compute stream, and `keep` holds what they read.
+// `stream` (the main stream when a pass is generic), and `keep` holds what
+// they read.
 template <typename R>
 void launch_window_passes_batch(const BatchItem* items, std::size_t count, uint32_t threads, BatchStats* stats,
                                 BatchKeep& keep, hipStream_t stream) {
