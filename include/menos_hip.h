@@ -304,12 +304,19 @@ int mh_batch_execute(const mh_batch_request* request, mh_batch_result* result);
 /* ---- Serving: a resident job server. Replaces the per-pub sequential
  * `run` loops of the reference's Python primitives (MENOSEstimator._run_pub,
  * python/menos/qiskit/estimator.py:91-146: one run per observable;
- * MENOSSampler._run_pub, sampler.py:93-129) for a stream of jobs. Jobs are
- * queued as they are submitted; a worker thread takes the queued jobs in
- * arrival order while their states fit the device budget (the ledger
- * admission, reference device_buffer.cpp:22-27), runs them as one batched
- * round (mh_batch_execute's kernels) and completes them. A job whose state
- * alone exceeds the budget completes with MH_ERR_RESOURCE.
+ * MENOSSampler._run_pub, sampler.py:93-129) for a stream of jobs.
+ *
+ * Continuous batching: jobs are queued as they are submitted, and several
+ * lanes (threads with streams of their own) take them. An idle lane admits
+ * the queued jobs, in arrival order, whose states fit the device memory the
+ * other lanes leave within the budget (the ledger admission, reference
+ * device_buffer.cpp:22-27), runs them as one batched round (mh_batch_execute's
+ * kernels) and completes them; jobs that arrive meanwhile are admitted by the
+ * next idle lane and their work overlaps on the device with the rounds
+ * already running. Within a round, jobs whose state is at most an eighth of
+ * its largest complete first. A job whose state alone exceeds the budget
+ * completes with MH_ERR_RESOURCE. A job's results do not depend on which
+ * lane or round ran it, nor on the other jobs beside it.
  *
  * Fusion plans are cached by circuit structure (opcodes, qubits, parameter
  * counts, unitary entries), so a parameter-only update of a circuit seen
@@ -323,7 +330,7 @@ int mh_batch_execute(const mh_batch_request* request, mh_batch_result* result);
 typedef struct mh_server mh_server;
 
 typedef struct mh_server_options {
-    uint64_t budget_bytes;          /* device bytes of one round's states; 0: 256 GB, capped to what the device has */
+    uint64_t budget_bytes;          /* device bytes the states of all running rounds may hold; 0: 256 GB, capped to what the device has */
     uint32_t max_round_jobs;        /* 0: no cap */
     uint32_t host_threads;          /* planning threads; 0: the hardware's, at most 16 */
     int32_t fusion;                 /* mh_fusion */
@@ -332,6 +339,7 @@ typedef struct mh_server_options {
     uint64_t reserve_bytes;         /* device bytes for round states allocated at creation (at most the
                                        budget); 0: the arena grows with the rounds */
     int32_t complex64_pairs;        /* nonzero: complex64 jobs use MH_FUSE_PAIRS (complex128 jobs keep `fusion`) */
+    uint32_t lanes;                 /* rounds that may run at once; 0: 3, at most 8 */
 } mh_server_options;
 
 typedef enum mh_job_kind { MH_JOB_ESTIMATOR = 0, MH_JOB_SAMPLER = 1 } mh_job_kind;
@@ -371,6 +379,7 @@ typedef struct mh_job_result {
     double submitted_seconds;       /* from the server's start */
     double started_seconds;         /* its round's start */
     double completed_seconds;       /* its results ready */
+    uint32_t lane;                  /* the lane that ran it (0-based) */
 } mh_job_result;
 
 typedef struct mh_server_stats {
@@ -379,15 +388,18 @@ typedef struct mh_server_stats {
     uint32_t rounds;
     uint64_t plan_cache_hits;
     uint64_t plan_cache_misses;
-    uint64_t peak_round_bytes;      /* the ledger's peak: the largest round's states */
+    uint64_t peak_round_bytes;      /* the largest round's states */
     uint64_t budget_bytes;          /* the budget in force */
-    double busy_seconds;            /* wall time inside rounds, split into: */
+    double busy_seconds;            /* wall time inside rounds, summed over lanes, split into: */
     double plan_seconds;            /*   host: fusion plans (or cache hits) and block matrices */
     double arena_seconds;           /*   growing the round-state arena */
     double compile_seconds;         /*   host: window-pass programs */
     double pass_seconds;            /*   device: initialisation and gate passes */
     double expectation_seconds;     /*   device: term reductions */
     double sample_seconds;          /*   device and host: sampler draws and counts */
+    uint32_t lanes;                 /* lanes in force */
+    uint32_t peak_concurrent_rounds;/* the most rounds that ran at once */
+    uint64_t peak_held_bytes;       /* the ledger's peak over all lanes */
 } mh_server_stats;
 
 int mh_server_create(const mh_server_options* options, mh_server** out);

@@ -91,7 +91,7 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
         if (it.num_terms > 0 && terms_out == nullptr) fail(MH_ERR_INVALID, "terms without an output");
         bytes += round_state_bytes(it.n, precision);
     }
-    Context& ctx = context();
+    context();
     DeviceBuffer own(arena_in ? 0 : bytes);
     unsigned char* const arena = static_cast<unsigned char*>(arena_in ? arena_in : own.get());
     const std::size_t count = items.size();
@@ -122,7 +122,7 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
         at += round_state_bytes(it.n, precision);
     }
     // Initialise: zeros, then each state's first amplitude.
-    MH_HIP(hipMemsetAsync(arena, 0, bytes, ctx.stream));
+    MH_HIP(hipMemsetAsync(arena, 0, bytes, streams().main));
     DeviceBuffer doffsets(offsets.size() * sizeof(uint64_t)), dphases(phases.size() * sizeof(double));
     // On an error past this point, wait for the queued work before the
     // buffers it uses go back to the pool.
@@ -132,17 +132,17 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
         ~WaitOnUnwind() {
             if (std::uncaught_exceptions() > pending) (void)hipStreamSynchronize(stream);
         }
-    } wait_on_unwind{ctx.stream};
+    } wait_on_unwind{streams().main};
     upload(doffsets.get(), offsets.data(), offsets.size() * sizeof(uint64_t));
     upload(dphases.get(), phases.data(), phases.size() * sizeof(double));
     const uint32_t n_here = static_cast<uint32_t>(count);
     const dim3 grid((n_here + kThreads - 1) / kThreads), block(kThreads);
     if (precision == MH_C128)
-        hipLaunchKernelGGL(first_amplitude_kernel<double2>, grid, block, 0, ctx.stream, reinterpret_cast<double2*>(arena),
+        hipLaunchKernelGGL(first_amplitude_kernel<double2>, grid, block, 0, streams().main, reinterpret_cast<double2*>(arena),
                            static_cast<const uint64_t*>(doffsets.as<uint64_t>()),
                            static_cast<const double*>(dphases.as<double>()), n_here);
     else
-        hipLaunchKernelGGL(first_amplitude_kernel<float2>, grid, block, 0, ctx.stream, reinterpret_cast<float2*>(arena),
+        hipLaunchKernelGGL(first_amplitude_kernel<float2>, grid, block, 0, streams().main, reinterpret_cast<float2*>(arena),
                            static_cast<const uint64_t*>(doffsets.as<uint64_t>()),
                            static_cast<const double*>(dphases.as<double>()), n_here);
     MH_HIP(hipGetLastError());

@@ -43,16 +43,16 @@ __global__ void __launch_bounds__(kThreads) fp64_kernel(double* out, uint32_t it
 
 template <typename F>
 double best_seconds(int reps, F&& f) {
-    Context& ctx = context();
+    context();
     hipEvent_t e0, e1;
     MH_HIP(hipEventCreate(&e0));
     MH_HIP(hipEventCreate(&e1));
     f();  // warm-up
     double best = 1e30;
     for (int i = 0; i < reps; ++i) {
-        MH_HIP(hipEventRecord(e0, ctx.stream));
+        MH_HIP(hipEventRecord(e0, streams().main));
         f();
-        MH_HIP(hipEventRecord(e1, ctx.stream));
+        MH_HIP(hipEventRecord(e1, streams().main));
         MH_HIP(hipEventSynchronize(e1));
         float ms = 0;
         MH_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -67,10 +67,10 @@ double best_seconds(int reps, F&& f) {
 
 // A streaming copy of `bytes` between two device buffers (bytes read + bytes written).
 void launch_copy(const void* src, void* dst, uint64_t bytes) {
-    Context& ctx = context();
+    context();
     const uint64_t n = bytes / sizeof(double2);
     const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((n + kThreads - 1) / kThreads, 1u << 20));
-    hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(kThreads), 0, ctx.stream, static_cast<const double2*>(src),
+    hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(kThreads), 0, streams().main, static_cast<const double2*>(src),
                        static_cast<double2*>(dst), n);
     MH_HIP(hipGetLastError());
 }
@@ -87,7 +87,7 @@ extern "C" int mh_calibrate(mh_calibration* out) {
         {
             const uint64_t bytes = uint64_t{1} << 31;  // 2 GiB each way
             DeviceBuffer a(bytes), b(bytes);
-            MH_HIP(hipMemsetAsync(a.get(), 0, bytes, ctx.stream));
+            MH_HIP(hipMemsetAsync(a.get(), 0, bytes, streams().main));
             const double s = best_seconds(5, [&] { launch_copy(a.get(), b.get(), bytes); });
             c.hbm_bytes_per_second = 2.0 * static_cast<double>(bytes) / s;
         }
@@ -108,7 +108,7 @@ extern "C" int mh_calibrate(mh_calibration* out) {
             c.pass_bytes_per_second = 2.0 * 16.0 * static_cast<double>(uint64_t{1} << n) * stats.passes / s;
         }
         c.launch_seconds = best_seconds(20, [&] {
-            hipLaunchKernelGGL(empty_kernel, dim3(1), dim3(64), 0, ctx.stream);
+            hipLaunchKernelGGL(empty_kernel, dim3(1), dim3(64), 0, streams().main);
             synchronize();
         });
         {
@@ -118,10 +118,10 @@ extern "C" int mh_calibrate(mh_calibration* out) {
             DeviceBuffer d(bytes);
             try {
                 const double h2d = best_seconds(3, [&] {
-                    MH_HIP(hipMemcpyAsync(d.get(), host, bytes, hipMemcpyHostToDevice, ctx.stream));
+                    MH_HIP(hipMemcpyAsync(d.get(), host, bytes, hipMemcpyHostToDevice, streams().main));
                 });
                 const double d2h = best_seconds(3, [&] {
-                    MH_HIP(hipMemcpyAsync(host, d.get(), bytes, hipMemcpyDeviceToHost, ctx.stream));
+                    MH_HIP(hipMemcpyAsync(host, d.get(), bytes, hipMemcpyDeviceToHost, streams().main));
                 });
                 c.h2d_bytes_per_second = static_cast<double>(bytes) / h2d;
                 c.d2h_bytes_per_second = static_cast<double>(bytes) / d2h;
@@ -136,7 +136,7 @@ extern "C" int mh_calibrate(mh_calibration* out) {
             const uint32_t iters = 4096;
             const uint32_t grid = static_cast<uint32_t>(ctx.props.multiProcessorCount) * 16;
             const double s = best_seconds(3, [&] {
-                hipLaunchKernelGGL(fp64_kernel, dim3(grid), dim3(kThreads), 0, ctx.stream, sink.as<double>(), iters,
+                hipLaunchKernelGGL(fp64_kernel, dim3(grid), dim3(kThreads), 0, streams().main, sink.as<double>(), iters,
                                    1.0000001, 1e-9);
             });
             c.fp64_ops_per_second = static_cast<double>(grid) * kThreads * iters * 16.0 / s;

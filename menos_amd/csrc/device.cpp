@@ -23,14 +23,25 @@ thread_local std::string g_last_error;
 
 void set_last_error(const std::string& message) { g_last_error = message; }
 
-void fail(mh_status status, const std::string& what) { throw Error(status, what); }
-
 void check_hip(hipError_t e, const char* what) {
     if (e != hipSuccess) {
         (void)hipGetLastError();
         fail(MH_ERR_INTERNAL, std::string(what) + ": " + hipGetErrorString(e));
     }
 }
+
+namespace {
+
+// The stream set installed by an OwnStreams on this thread, if any.
+thread_local Streams* t_streams = nullptr;
+
+void create_streams(Streams& s) {
+    MH_HIP(hipStreamCreateWithFlags(&s.main, hipStreamNonBlocking));
+    MH_HIP(hipStreamCreateWithFlags(&s.second, hipStreamNonBlocking));
+    MH_HIP(hipStreamCreateWithFlags(&s.copy, hipStreamNonBlocking));
+}
+
+}  // namespace
 
 Context& context() {
     static Context* ctx = [] {
@@ -45,16 +56,34 @@ Context& context() {
         if (std::strncmp(c->props.gcnArchName, "gfx950", 6) != 0) {
             fail(MH_ERR_DEVICE, std::string("device is ") + c->props.gcnArchName + ", this library is built for gfx950");
         }
-        MH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        MH_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
-        MH_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+        create_streams(c->defaults);
         return c;
     }();
     MH_HIP(hipSetDevice(ctx->device));
     return *ctx;
 }
 
-void synchronize() { MH_HIP(hipStreamSynchronize(context().stream)); }
+Streams& streams() {
+    Context& ctx = context();
+    return t_streams ? *t_streams : ctx.defaults;
+}
+
+OwnStreams::OwnStreams() {
+    context();
+    create_streams(own_);
+    previous_ = t_streams;
+    t_streams = &own_;
+}
+
+OwnStreams::~OwnStreams() {
+    t_streams = previous_;
+    for (hipStream_t s : {own_.copy, own_.second, own_.main}) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
+}
+
+void synchronize() { MH_HIP(hipStreamSynchronize(streams().main)); }
 
 std::size_t amp_bytes(int32_t precision) {
     if (precision == MH_C128) return 16;
@@ -168,15 +197,15 @@ void pool_trim() {
 
 void upload(void* device_dst, const void* host_src, std::size_t bytes) {
     if (bytes == 0) return;
-    Context& ctx = context();
-    MH_HIP(hipMemcpyAsync(device_dst, host_src, bytes, hipMemcpyHostToDevice, ctx.stream));
+    context();
+    MH_HIP(hipMemcpyAsync(device_dst, host_src, bytes, hipMemcpyHostToDevice, streams().main));
 }
 
 void download(void* host_dst, const void* device_src, std::size_t bytes) {
     if (bytes == 0) return;
-    Context& ctx = context();
-    MH_HIP(hipMemcpyAsync(host_dst, device_src, bytes, hipMemcpyDeviceToHost, ctx.stream));
-    MH_HIP(hipStreamSynchronize(ctx.stream));
+    context();
+    MH_HIP(hipMemcpyAsync(host_dst, device_src, bytes, hipMemcpyDeviceToHost, streams().main));
+    MH_HIP(hipStreamSynchronize(streams().main));
 }
 
 }  // namespace mh

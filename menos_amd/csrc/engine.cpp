@@ -157,7 +157,7 @@ ExecutionResult HipDenseEngine::execute(const ExecutionRequest& r) const {
     if (r.budget_bytes != 0 && need > r.budget_bytes)
         fail(MH_ERR_RESOURCE, "the dense state needs " + std::to_string(need) + " device bytes, budget is " +
                                   std::to_string(r.budget_bytes));
-    Context& ctx = context();
+    context();
 
     ExecutionResult result;
     result.num_blocks = static_cast<uint32_t>(p.blocks.size());
@@ -171,11 +171,11 @@ ExecutionResult HipDenseEngine::execute(const ExecutionRequest& r) const {
         hipEvent_t g0, g1;
         MH_HIP(hipEventCreate(&g0));
         MH_HIP(hipEventCreate(&g1));
-        MH_HIP(hipEventRecord(g0, ctx.stream));
+        MH_HIP(hipEventRecord(g0, streams().main));
         PassStats stats;
         apply_blocks(state.get(), n, r.precision, p.blocks, window_for(r), r.record_pass_times, &stats,
                      r.max_pass_cost);
-        MH_HIP(hipEventRecord(g1, ctx.stream));
+        MH_HIP(hipEventRecord(g1, streams().main));
         synchronize();
         float gate_ms = 0;
         MH_HIP(hipEventElapsedTime(&gate_ms, g0, g1));

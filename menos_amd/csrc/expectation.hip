@@ -427,7 +427,7 @@ void expectation_terms(const void* state, uint32_t n, int32_t precision, const u
 // one finishing launch over every term.
 template <typename V>
 void expectation_batch_typed(const std::vector<ExpectationItem>& items, double* out) {
-    Context& ctx = context();
+    context();
     uintptr_t base_addr = UINTPTR_MAX;
     for (const ExpectationItem& it : items) base_addr = std::min(base_addr, reinterpret_cast<uintptr_t>(it.state));
     std::vector<ExpJob> jobs;
@@ -564,7 +564,7 @@ void expectation_batch_typed(const std::vector<ExpectationItem>& items, double* 
         upload(dtjobs.get(), tjobs.data(), tjobs.size() * sizeof(TileJob));
         upload(dtgroups.get(), tterms.data(), tterms.size() * sizeof(TileTerm));
         upload(dtbegin.get(), tblock_begin.data(), tblock_begin.size() * sizeof(uint32_t));
-        hipLaunchKernelGGL(expectation_tile_kernel<V>, dim3(tile_blocks), dim3(kThreads), 0, ctx.stream,
+        hipLaunchKernelGGL(expectation_tile_kernel<V>, dim3(tile_blocks), dim3(kThreads), 0, streams().main,
                            static_cast<const TileJob*>(dtjobs.as<TileJob>()),
                            static_cast<const uint32_t*>(dtbegin.as<uint32_t>()), static_cast<uint32_t>(tjobs.size()),
                            static_cast<const TileTerm*>(dtgroups.as<TileTerm>()), reinterpret_cast<const V*>(base_addr),
@@ -576,13 +576,13 @@ void expectation_batch_typed(const std::vector<ExpectationItem>& items, double* 
         dbegin = DeviceBuffer(block_begin.size() * sizeof(uint32_t));
         upload(djobs.get(), jobs.data(), jobs.size() * sizeof(ExpJob));
         upload(dbegin.get(), block_begin.data(), block_begin.size() * sizeof(uint32_t));
-        hipLaunchKernelGGL(expectation_batch_kernel<V>, dim3(total_blocks), dim3(kThreads), 0, ctx.stream,
+        hipLaunchKernelGGL(expectation_batch_kernel<V>, dim3(total_blocks), dim3(kThreads), 0, streams().main,
                            static_cast<const ExpJob*>(djobs.as<ExpJob>()), static_cast<const uint32_t*>(dbegin.as<uint32_t>()),
                            static_cast<uint32_t>(jobs.size()), reinterpret_cast<const V*>(base_addr), dpartials.as<double>());
         MH_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(expectation_finish_batch_kernel, dim3(static_cast<uint32_t>(slots.size())), dim3(kThreads), 0,
-                       ctx.stream, static_cast<const ExpSlot*>(dslots.as<ExpSlot>()),
+                       streams().main, static_cast<const ExpSlot*>(dslots.as<ExpSlot>()),
                        static_cast<const double*>(dpartials.as<double>()), dout.as<double>());
     MH_HIP(hipGetLastError());
     std::vector<double> host(uint64_t(out_terms) * 2);
@@ -626,10 +626,10 @@ __global__ void __launch_bounds__(kThreads) max_diff_kernel(const VA* a, const V
 
 template <typename VA, typename VB>
 double max_diff(const void* a, const void* b, uint64_t size) {
-    Context& ctx = context();
+    context();
     const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(4096, std::max<uint64_t>(1, size / kThreads)));
     DeviceBuffer partials(grid * sizeof(double));
-    hipLaunchKernelGGL((max_diff_kernel<VA, VB>), dim3(grid), dim3(kThreads), 0, ctx.stream, static_cast<const VA*>(a),
+    hipLaunchKernelGGL((max_diff_kernel<VA, VB>), dim3(grid), dim3(kThreads), 0, streams().main, static_cast<const VA*>(a),
                        static_cast<const VB*>(b), size, partials.as<double>());
     MH_HIP(hipGetLastError());
     std::vector<double> host(grid);

@@ -123,7 +123,7 @@ uint32_t grid_for(uint64_t items) {
 template <typename R>
 void run_generic(void* state, uint32_t n, const cplx* matrix, const uint32_t* operands, uint32_t k) {
     using V = typename Vec<R>::type;
-    Context& ctx = context();
+    context();
     const std::size_t dim = std::size_t{1} << k;
     std::vector<V> m(dim * dim);
     for (std::size_t e = 0; e < dim * dim; ++e) {
@@ -142,12 +142,12 @@ void run_generic(void* state, uint32_t n, const cplx* matrix, const uint32_t* op
     const std::size_t lds = sizeof(V) << (2 * k);
     const dim3 grid(grid_for(a.groups)), block(kThreads);
     switch (k) {
-        case 1: hipLaunchKernelGGL((generic_gate_kernel<R, 1>), grid, block, lds, ctx.stream, a); break;
-        case 2: hipLaunchKernelGGL((generic_gate_kernel<R, 2>), grid, block, lds, ctx.stream, a); break;
-        case 3: hipLaunchKernelGGL((generic_gate_kernel<R, 3>), grid, block, lds, ctx.stream, a); break;
-        case 4: hipLaunchKernelGGL((generic_gate_kernel<R, 4>), grid, block, lds, ctx.stream, a); break;
-        case 5: hipLaunchKernelGGL((generic_gate_kernel<R, 5>), grid, block, lds, ctx.stream, a); break;
-        default: hipLaunchKernelGGL((generic_gate_kernel<R, 6>), grid, block, lds, ctx.stream, a); break;
+        case 1: hipLaunchKernelGGL((generic_gate_kernel<R, 1>), grid, block, lds, streams().main, a); break;
+        case 2: hipLaunchKernelGGL((generic_gate_kernel<R, 2>), grid, block, lds, streams().main, a); break;
+        case 3: hipLaunchKernelGGL((generic_gate_kernel<R, 3>), grid, block, lds, streams().main, a); break;
+        case 4: hipLaunchKernelGGL((generic_gate_kernel<R, 4>), grid, block, lds, streams().main, a); break;
+        case 5: hipLaunchKernelGGL((generic_gate_kernel<R, 5>), grid, block, lds, streams().main, a); break;
+        default: hipLaunchKernelGGL((generic_gate_kernel<R, 6>), grid, block, lds, streams().main, a); break;
     }
     MH_HIP(hipGetLastError());
     synchronize();
@@ -174,12 +174,12 @@ bool matrix_is_diagonal(const cplx* m, std::size_t dim) {
 }
 
 void launch_init(void* state, uint32_t n, int32_t precision, double re, double im) {
-    Context& ctx = context();
+    context();
     const uint64_t size = uint64_t{1} << n;
     if (precision == MH_C128)
-        hipLaunchKernelGGL(init_kernel<double>, dim3(grid_for(size)), dim3(kThreads), 0, ctx.stream, state, size, re, im);
+        hipLaunchKernelGGL(init_kernel<double>, dim3(grid_for(size)), dim3(kThreads), 0, streams().main, state, size, re, im);
     else
-        hipLaunchKernelGGL(init_kernel<float>, dim3(grid_for(size)), dim3(kThreads), 0, ctx.stream, state, size,
+        hipLaunchKernelGGL(init_kernel<float>, dim3(grid_for(size)), dim3(kThreads), 0, streams().main, state, size,
                            static_cast<float>(re), static_cast<float>(im));
     MH_HIP(hipGetLastError());
 }

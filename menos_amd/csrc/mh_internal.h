@@ -10,18 +10,12 @@
 #include <vector>
 
 #include "menos_hip.h"
+#include "mh_error.h"
 
 namespace mh {
 
 using cplx = std::complex<double>;
 
-// Errors raised inside the library and turned into statuses at the C boundary.
-struct Error : std::runtime_error {
-    mh_status status;
-    Error(mh_status s, const std::string& what) : std::runtime_error(what), status(s) {}
-};
-
-[[noreturn]] void fail(mh_status status, const std::string& what);
 void check_hip(hipError_t e, const char* what);
 #define MH_HIP(call) ::mh::check_hip((call), #call)
 
@@ -45,16 +39,43 @@ int guarded(F&& f) noexcept {
     }
 }
 
-// The library's device context: three streams of default priority on device 0,
-// created on first use.
+// A set of three streams of default priority: every kernel and copy the
+// library issues goes to the calling thread's set.
+struct Streams {
+    hipStream_t main = nullptr;    // kernels
+    hipStream_t second = nullptr;  // a second kernel stream: batched groups alternate between the two
+    hipStream_t copy = nullptr;    // table uploads that run ahead of the kernels
+};
+
+// The library's device context on device 0, created on first use, with the
+// stream set of every thread that has none of its own.
 struct Context {
     int device = 0;
-    hipStream_t stream = nullptr;       // kernels
-    hipStream_t stream2 = nullptr;      // a second kernel stream: batched groups alternate between the two
-    hipStream_t copy_stream = nullptr;  // table uploads that run ahead of the kernels
+    Streams defaults;
     hipDeviceProp_t props{};
 };
 Context& context();
+
+// The calling thread's streams: those an OwnStreams on this thread installed,
+// else the context's.
+Streams& streams();
+
+// Gives the calling thread a stream set of its own for the scope's lifetime,
+// so that its work overlaps on the device with other threads' (the job
+// server's lanes). The destructor waits for the set's work, then destroys it.
+class OwnStreams {
+public:
+    OwnStreams();
+    ~OwnStreams();
+    OwnStreams(const OwnStreams&) = delete;
+    OwnStreams& operator=(const OwnStreams&) = delete;
+
+private:
+    Streams own_;
+    Streams* previous_ = nullptr;
+};
+
+// Waits for the calling thread's main stream.
 void synchronize();
 
 std::size_t amp_bytes(int32_t precision);

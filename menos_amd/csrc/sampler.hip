@@ -408,7 +408,7 @@ __global__ void __launch_bounds__(kThreads) shots_kernel(SampleArgs a, const dou
 
 template <typename V>
 void run_sampler(const void* state, uint32_t n, const double* sorted_u, uint64_t shots, uint64_t* basis_out) {
-    Context& ctx = context();
+    context();
     SampleArgs a{};
     a.state = state;
     a.size = uint64_t{1} << n;
@@ -436,21 +436,21 @@ void run_sampler(const void* state, uint32_t n, const double* sorted_u, uint64_t
     uint64_t* dout = reinterpret_cast<uint64_t*>(take(shots * sizeof(uint64_t)));
     upload(du, sorted_u, shots * sizeof(double));
 
-    hipLaunchKernelGGL(block_sums_kernel<V>, dim3(a.blocks), dim3(kThreads), 0, ctx.stream, a);
+    hipLaunchKernelGGL(block_sums_kernel<V>, dim3(a.blocks), dim3(kThreads), 0, streams().main, a);
     MH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(approx_prefix_kernel, dim3(1), dim3(1024), 0, ctx.stream, a);
+    hipLaunchKernelGGL(approx_prefix_kernel, dim3(1), dim3(1024), 0, streams().main, a);
     MH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(block_xfer_kernel<V>, dim3(a.blocks), dim3(kThreads), 0, ctx.stream, a);
+    hipLaunchKernelGGL(block_xfer_kernel<V>, dim3(a.blocks), dim3(kThreads), 0, streams().main, a);
     MH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(group_xfer_kernel, dim3(a.groups), dim3(kThreads), 0, ctx.stream, a);
+    hipLaunchKernelGGL(group_xfer_kernel, dim3(a.groups), dim3(kThreads), 0, streams().main, a);
     MH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(chain_kernel<V>, dim3(1), dim3(kThreads), 0, ctx.stream, a);
+    hipLaunchKernelGGL(chain_kernel<V>, dim3(1), dim3(kThreads), 0, streams().main, a);
     MH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(expand_kernel, dim3(a.groups), dim3(kThreads), 0, ctx.stream, a);
+    hipLaunchKernelGGL(expand_kernel, dim3(a.groups), dim3(kThreads), 0, streams().main, a);
     MH_HIP(hipGetLastError());
     if (shots > 0) {
         const uint32_t grid = static_cast<uint32_t>((shots + kThreads - 1) / kThreads);
-        hipLaunchKernelGGL(shots_kernel<V>, dim3(grid), dim3(kThreads), 0, ctx.stream, a,
+        hipLaunchKernelGGL(shots_kernel<V>, dim3(grid), dim3(kThreads), 0, streams().main, a,
                            static_cast<const double*>(du), shots, dout);
         MH_HIP(hipGetLastError());
     }

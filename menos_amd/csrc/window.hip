@@ -1105,7 +1105,7 @@ void launch_window_passes(void* state, uint32_t n, const std::vector<Block>& blo
                           const std::vector<WindowPass>& passes, bool time_passes, PassStats* stats) {
     using V = typename Shape<R>::V;
     constexpr uint32_t W = width_of<R>();
-    Context& ctx = context();
+    context();
     std::vector<WPassArgs> args(passes.size());
     std::vector<uint32_t> bundle_begin(passes.size(), 0), op_begin(passes.size(), 0);
     std::vector<uint64_t> bundles;
@@ -1134,7 +1134,7 @@ void launch_window_passes(void* state, uint32_t n, const std::vector<Block>& blo
     if (time_passes) {
         events.resize(passes.size() + 1);
         for (auto& e : events) MH_HIP(hipEventCreate(&e));
-        MH_HIP(hipEventRecord(events[0], ctx.stream));
+        MH_HIP(hipEventRecord(events[0], streams().main));
     }
     for (std::size_t p = 0; p < passes.size(); ++p) {
         if (passes[p].generic) {
@@ -1153,12 +1153,12 @@ void launch_window_passes(void* state, uint32_t n, const std::vector<Block>& blo
             const uint32_t group_log = std::min<uint32_t>(waves_log, 2);
             const dim3 grid(1u << (waves_log - group_log)), block(64u << group_log);
             if (uses_pairs(bundles.data() + bundle_begin[p], args[p].lay.num_bundles, Shape<R>::kRegBits))
-                hipLaunchKernelGGL((window_kernel<R, true>), grid, block, 0, ctx.stream, a);
+                hipLaunchKernelGGL((window_kernel<R, true>), grid, block, 0, streams().main, a);
             else
-                hipLaunchKernelGGL((window_kernel<R, false>), grid, block, 0, ctx.stream, a);
+                hipLaunchKernelGGL((window_kernel<R, false>), grid, block, 0, streams().main, a);
             MH_HIP(hipGetLastError());
         }
-        if (time_passes) MH_HIP(hipEventRecord(events[p + 1], ctx.stream));
+        if (time_passes) MH_HIP(hipEventRecord(events[p + 1], streams().main));
     }
     synchronize();
     if (time_passes) {
@@ -1195,7 +1195,7 @@ void launch_window_passes_batch(const BatchItem* items, std::size_t count, uint3
                                 BatchKeep& keep, hipStream_t stream) {
     using V = typename Shape<R>::V;
     constexpr uint32_t W = width_of<R>();
-    Context& ctx = context();
+    context();
     struct PassRec {
         WPassArgs args;
         uint32_t bundle_begin = 0, op_begin = 0;
@@ -1282,10 +1282,10 @@ void launch_window_passes_batch(const BatchItem* items, std::size_t count, uint3
     const WOp* const dops_p = dops.as<WOp>();
     const SignTerm* const dterms_p = dterms.as<SignTerm>();
     const V* const dmats_p = dmats.as<V>();
-    keep_upload(keep, dbundles, std::move(bundles), ctx.copy_stream);
-    keep_upload(keep, dops, std::move(ops), ctx.copy_stream);
-    keep_upload(keep, dterms, std::move(terms), ctx.copy_stream);
-    keep_upload(keep, dmats, std::move(mats), ctx.copy_stream);
+    keep_upload(keep, dbundles, std::move(bundles), streams().copy);
+    keep_upload(keep, dops, std::move(ops), streams().copy);
+    keep_upload(keep, dterms, std::move(terms), streams().copy);
+    keep_upload(keep, dmats, std::move(mats), streams().copy);
 
     // Launch lists: pass p of every state that has one. State offsets count
     // amplitudes from the lowest state address.
@@ -1327,16 +1327,16 @@ void launch_window_passes_batch(const BatchItem* items, std::size_t count, uint3
     DeviceBuffer dgroups(std::max<std::size_t>(all_groups.size(), 1) * sizeof(uint32_t));
     const WBatchArgs* const dargs_p = dargs.as<WBatchArgs>();
     const uint32_t* const dgroups_p = dgroups.as<uint32_t>();
-    keep_upload(keep, dargs, std::move(all_args), ctx.copy_stream);
-    keep_upload(keep, dgroups, std::move(all_groups), ctx.copy_stream);
+    keep_upload(keep, dargs, std::move(all_args), streams().copy);
+    keep_upload(keep, dgroups, std::move(all_groups), streams().copy);
     MH_HIP(hipEventCreateWithFlags(&keep.copied, hipEventDisableTiming));
-    MH_HIP(hipEventRecord(keep.copied, ctx.copy_stream));
+    MH_HIP(hipEventRecord(keep.copied, streams().copy));
     // A generic pass runs on the main stream (launch_generic), so a group that
     // has one stays there in full.
     bool has_generic = false;
     for (std::size_t i = 0; i < count && !has_generic; ++i)
         for (const PassRec& rec : comp[i].passes) has_generic = has_generic || rec.generic;
-    if (has_generic) stream = ctx.stream;
+    if (has_generic) stream = streams().main;
     MH_HIP(hipStreamWaitEvent(stream, keep.copied, 0));
     keep.device.push_back(std::move(dbundles));
     keep.device.push_back(std::move(dops));
@@ -1446,18 +1446,18 @@ BatchPassRunner::BatchPassRunner(int32_t precision, uint32_t threads) : impl_(st
     amp_bytes(precision);
     impl_->precision = precision;
     impl_->threads = threads;
-    Context& ctx = context();
+    context();
     MH_HIP(hipEventCreateWithFlags(&impl_->start, hipEventDisableTiming));
-    MH_HIP(hipEventRecord(impl_->start, ctx.stream));
-    MH_HIP(hipStreamWaitEvent(ctx.stream2, impl_->start, 0));
+    MH_HIP(hipEventRecord(impl_->start, streams().main));
+    MH_HIP(hipStreamWaitEvent(streams().second, impl_->start, 0));
 }
 
 BatchPassRunner::~BatchPassRunner() {
     // Whatever was launched or copied reads the kept tables: wait before they go.
     if (!impl_->keeps.empty()) {
-        (void)hipStreamSynchronize(context().copy_stream);
-        (void)hipStreamSynchronize(context().stream2);
-        (void)hipStreamSynchronize(context().stream);
+        (void)hipStreamSynchronize(streams().copy);
+        (void)hipStreamSynchronize(streams().second);
+        (void)hipStreamSynchronize(streams().main);
     }
     for (BatchKeep& k : impl_->keeps)
         if (k.copied) (void)hipEventDestroy(k.copied);
@@ -1472,8 +1472,8 @@ void BatchPassRunner::add(const BatchItem* items, std::size_t count, BatchStats*
     impl_->keeps.emplace_back();
     // Groups alternate between the two kernel streams, so the short launches at
     // the end of one group overlap the next group's.
-    Context& ctx = context();
-    const hipStream_t stream = (impl_->added++ % 2 == 0) ? ctx.stream : ctx.stream2;
+    context();
+    const hipStream_t stream = (impl_->added++ % 2 == 0) ? streams().main : streams().second;
     if (impl_->precision == MH_C128)
         launch_window_passes_batch<double>(items, count, impl_->threads, stats, impl_->keeps.back(), stream);
     else
@@ -1481,8 +1481,8 @@ void BatchPassRunner::add(const BatchItem* items, std::size_t count, BatchStats*
 }
 
 void BatchPassRunner::finish() {
-    MH_HIP(hipStreamSynchronize(context().copy_stream));
-    MH_HIP(hipStreamSynchronize(context().stream2));
+    MH_HIP(hipStreamSynchronize(streams().copy));
+    MH_HIP(hipStreamSynchronize(streams().second));
     synchronize();
     for (BatchKeep& k : impl_->keeps)
         if (k.copied) (void)hipEventDestroy(k.copied);

@@ -381,7 +381,7 @@ class CServerOptions(ctypes.Structure):
     _fields_ = [("budget_bytes", ctypes.c_uint64), ("max_round_jobs", ctypes.c_uint32),
                 ("host_threads", ctypes.c_uint32), ("fusion", ctypes.c_int32), ("fusion_width", ctypes.c_uint32),
                 ("plan_cache_capacity", ctypes.c_uint32), ("reserve_bytes", ctypes.c_uint64),
-                ("complex64_pairs", ctypes.c_int32)]
+                ("complex64_pairs", ctypes.c_int32), ("lanes", ctypes.c_uint32)]
 
 
 class CJob(ctypes.Structure):
@@ -397,7 +397,7 @@ class CJobResult(ctypes.Structure):
                 ("status", ctypes.c_int32), ("error", ctypes.c_char * 256), ("num_outcomes", ctypes.c_uint64),
                 ("round", ctypes.c_uint32), ("plan_cache_hit", ctypes.c_int32),
                 ("submitted_seconds", ctypes.c_double), ("started_seconds", ctypes.c_double),
-                ("completed_seconds", ctypes.c_double)]
+                ("completed_seconds", ctypes.c_double), ("lane", ctypes.c_uint32)]
 
 
 class CServerStats(ctypes.Structure):
@@ -406,7 +406,9 @@ class CServerStats(ctypes.Structure):
                 ("peak_round_bytes", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64),
                 ("busy_seconds", ctypes.c_double), ("plan_seconds", ctypes.c_double), ("arena_seconds", ctypes.c_double),
                 ("compile_seconds", ctypes.c_double), ("pass_seconds", ctypes.c_double),
-                ("expectation_seconds", ctypes.c_double), ("sample_seconds", ctypes.c_double)]
+                ("expectation_seconds", ctypes.c_double), ("sample_seconds", ctypes.c_double),
+                ("lanes", ctypes.c_uint32), ("peak_concurrent_rounds", ctypes.c_uint32),
+                ("peak_held_bytes", ctypes.c_uint64)]
 
 
 def derive_seeds(seed: int, count: int) -> list[int]:
@@ -463,7 +465,8 @@ class Job:
 
     def collect(self, r: CJobResult, bufs) -> Result:
         vals, raw, outc, cnts = bufs
-        out = Result(status=r.status, error=r.error.decode(), round=r.round, plan_cache_hit=bool(r.plan_cache_hit),
+        out = Result(status=r.status, error=r.error.decode(), round=r.round, lane=r.lane,
+                     plan_cache_hit=bool(r.plan_cache_hit),
                      submitted=r.submitted_seconds, started=r.started_seconds, completed=r.completed_seconds,
                      latency=r.completed_seconds - r.submitted_seconds)
         if r.status == MH_OK:
@@ -478,9 +481,9 @@ class Server:
     """A resident job server (mh_server_*): submit jobs, wait for results."""
 
     def __init__(self, *, budget_bytes=0, max_round_jobs=0, host_threads=0, fusion=FUSE_CONTAINMENT,
-                 fusion_width=0, plan_cache_capacity=0, reserve_bytes=0, complex64_pairs=False):
+                 fusion_width=0, plan_cache_capacity=0, reserve_bytes=0, complex64_pairs=False, lanes=0):
         o = CServerOptions(budget_bytes, max_round_jobs, host_threads, fusion, fusion_width, plan_cache_capacity,
-                           reserve_bytes, int(bool(complex64_pairs)))
+                           reserve_bytes, int(bool(complex64_pairs)), lanes)
         self._h = ctypes.c_void_p()
         check(lib().mh_server_create(ctypes.byref(o), ctypes.byref(self._h)))
         self._pending = {}

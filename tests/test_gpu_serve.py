@@ -133,6 +133,48 @@ def test_replay_gives_the_closed_loop_results(hip):
             assert a.counts == b.counts
 
 
+def test_small_jobs_waited_first_beside_a_large_one(hip):
+    """One large job and several at most an eighth of its size in one round:
+    the small ones are waited for first, while the large one may still run;
+    every job completes once, with the single engine's results."""
+    big_c, big_obs, big = _estimator(hip, 20, 3, 500)
+    small = [_estimator(hip, 10 + i % 3, 2, 510 + i) for i in range(6)]
+    with hip.Server(lanes=1) as s:
+        t_big = s.submit(big)
+        t_small = [s.submit(j) for _, _, j in small]
+        r_small = [s.wait(t) for t in t_small]
+        r_big = s.wait(t_big)
+        st = s.stats()
+    assert st.jobs_completed == 7 and st.jobs_failed == 0
+    for (c, obs, _), r in zip(small, r_small):
+        _check_estimator(hip, c, obs, r)
+    _check_estimator(hip, big_c, big_obs, r_big)
+
+
+def test_arrivals_overlap_a_running_round(hip):
+    """Continuous batching: jobs that arrive while a long round runs are
+    admitted by another lane, run beside it and complete first; each result is
+    the one the job gets alone."""
+    big_c, big_j = _sampler(hip, 28, 12, 600)     # 4 GiB, tens of milliseconds of passes
+    small = [_estimator(hip, 12, 3, 610 + i) for i in range(8)]
+    jobs = [big_j] + [j for _, _, j in small]
+    arrivals = [0.0] + [0.005 + 0.003 * i for i in range(len(small))]
+    with hip.Server(lanes=3) as s:
+        res = s.replay(jobs, arrivals)
+        st = s.stats()
+    assert st.jobs_completed == len(jobs) and st.jobs_failed == 0
+    assert st.lanes == 3 and st.peak_concurrent_rounds >= 2
+    r_big = res[0]
+    assert r_big.status == 0, r_big.error
+    beside = [r for r in res[1:] if r.lane != r_big.lane]
+    assert beside and any(r.completed < r_big.completed for r in beside)
+    for (c, obs, _), r in zip(small, res[1:]):
+        _check_estimator(hip, c, obs, r)
+    single = hip.HipDenseEngine().execute(big_c, shots=big_j.shots, seed=big_j.c.seed, targets=big_c.measure_targets(),
+                                          width=big_c.num_qubits, statevector=False)
+    assert r_big.counts == single.counts
+
+
 @pytest.mark.parametrize("pairs", [False, True])
 def test_complex64_estimator_error(hip, pairs):
     """complex64 estimator jobs (under the containment plan or, with
