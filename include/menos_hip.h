@@ -42,7 +42,8 @@ typedef enum mh_status {
     MH_ERR_INVALID = 1,  /* a precondition of the call does not hold        */
     MH_ERR_RESOURCE = 2, /* over budget or out of device memory (bytes named) */
     MH_ERR_INTERNAL = 3, /* a HIP call failed                                */
-    MH_ERR_DEVICE = 4    /* no usable gfx950 device                          */
+    MH_ERR_DEVICE = 4,   /* no usable gfx950 device                          */
+    MH_ERR_ABANDONED = 5 /* the execution monitor asked to stop               */
 } mh_status;
 
 typedef enum mh_precision { MH_C128 = 0, MH_C64 = 1 } mh_precision;
@@ -86,8 +87,10 @@ int mh_pool_trim(void);
 int mh_state_init(void* state, uint32_t num_qubits, int32_t precision, double phase_re, double phase_im);
 
 /* One gate: a 2^k x 2^k matrix (host memory, 2*4^k doubles) at `operands`
- * (host memory), k <= 6. Diagonal matrices take the streaming diagonal kernel,
- * k = 1 and 2 their own kernels; every path is bitwise the generic kernel. */
+ * (host memory), k <= 6, as one HBM pass. Diagonal blocks and blocks of 1 or 2
+ * qubits run in the window-pass kernel (register windows, sign and diagonal
+ * stages, controlled pairs), wider dense blocks in the per-group kernel; every
+ * path is bitwise the generic kernel. */
 int mh_apply_matrix(void* state, uint32_t num_qubits, int32_t precision, const double* matrix,
                     const uint32_t* operands, uint32_t k);
 
@@ -160,8 +163,9 @@ typedef struct mh_circuit {
 } mh_circuit;
 
 /* How the gate list becomes blocks. MH_FUSE_NONE: one block per gate.
- * MH_FUSE_CONTAINMENT: the reference's rule (src/ir/fuse_gates.cpp:190-283),
- * its default. MH_FUSE_PAIRS: the containment rule, except that a gate does not
+ * MH_FUSE_CONTAINMENT: the reference's rule (plan_fusion,
+ * src/ir/fuse_gates.cpp:108-201, with may_join at :53-65), its default.
+ * MH_FUSE_PAIRS: the containment rule, except that a gate does not
  * join a block containing a CX if it would act non-diagonally on that CX's
  * control (nor a CX whose control such a gate touches), so those blocks stay
  * controlled pairs (a 2x2 block on the target per control value), which cost
@@ -171,6 +175,18 @@ typedef struct mh_circuit {
  * operands at most, 2..6), order-preserving; the oracle applies the same
  * block list. */
 typedef enum mh_fusion { MH_FUSE_NONE = 0, MH_FUSE_CONTAINMENT = 1, MH_FUSE_WIDTH = 2, MH_FUSE_PAIRS = 3 } mh_fusion;
+
+/* The execution monitor (reference ExecutionMonitor::at_safe_boundary,
+ * include/menos/simulation_engine.h:54-116, which apply_blocks calls after
+ * every block, src/engines/dense/fused_block.h:96-99). mh_engine_execute calls
+ * it after each HBM pass, on the calling thread, with the number of passes
+ * completed so far (from 1): there the state holds every block of those passes
+ * and nothing has been handed to the caller. The reference abandons by
+ * throwing; across this ABI a nonzero return abandons instead: the state is
+ * freed and the call returns MH_ERR_ABANDONED (the adapter rethrows what its
+ * monitor threw). The next pass may already be queued when it runs, and is
+ * waited for before the state is freed. */
+typedef int (*mh_monitor_fn)(void* user_data, uint64_t boundaries_crossed);
 
 typedef struct mh_execution_request {
     const mh_circuit* circuit;
@@ -192,6 +208,8 @@ typedef struct mh_execution_request {
     int32_t want_statevector;
     int32_t record_pass_times;     /* per-pass device times into the result */
     uint32_t max_pass_cost;        /* FP operations per amplitude one HBM pass may carry; 0 = default */
+    mh_monitor_fn monitor;         /* null: no monitor */
+    void* monitor_data;
 } mh_execution_request;
 
 typedef struct mh_execution_result {
@@ -231,7 +249,12 @@ typedef struct mh_cost_estimate {
 } mh_cost_estimate;
 
 /* Never fails on an infeasible request: it reports feasible = 0 and a reason
- * (reference simulation_engine.h:349-359). */
+ * (reference simulation_engine.h:349-359). Host work only (plans the blocks and
+ * passes, applies the time model); it needs no device. The predicted seconds
+ * are the calibrated profile's:
+ *   launches * launch_seconds + passes * 2 * state / pass_bytes_per_second
+ *   + state / hbm (initialisation) + 3 * state / hbm if sampling
+ *   + terms * state / hbm + h2d / h2d_rate + d2h / d2h_rate. */
 int mh_engine_estimate(const mh_execution_request* request, mh_cost_estimate* out);
 int mh_engine_execute(const mh_execution_request* request, mh_execution_result* result);
 
@@ -258,7 +281,14 @@ typedef struct mh_calibration {
     double fp64_ops_per_second;      /* non-fused DMUL/DADD */
 } mh_calibration;
 
+/* Measures the device (about a second) and installs the result as the time
+ * model's profile. */
 int mh_calibrate(mh_calibration* out);
+/* Installs a profile measured earlier or made up for a test (the reference's
+ * planner takes its HardwareProfile the same way, hardware_profile.h:52-77),
+ * and reads the one in force. Neither touches the device. */
+int mh_set_calibration(const mh_calibration* profile);
+int mh_get_calibration(mh_calibration* out);
 
 /* ---- Batched execution: many independent circuits per device launch.
  * Replaces the reference's sequential batch (run_batch_with_options,

@@ -168,19 +168,30 @@ ExecutionResult HipDenseEngine::execute(const ExecutionRequest& r) const {
         const std::complex<double> phase = host_phase(r.global_phase);
         launch_init(state.get(), n, r.precision, phase.real(), phase.imag());
 
-        hipEvent_t g0, g1;
+        hipEvent_t g0 = nullptr, g1 = nullptr;
+        struct EventPair {
+            hipEvent_t& a;
+            hipEvent_t& b;
+            ~EventPair() {
+                if (a) (void)hipEventDestroy(a);
+                if (b) (void)hipEventDestroy(b);
+            }
+        } events{g0, g1};
         MH_HIP(hipEventCreate(&g0));
         MH_HIP(hipEventCreate(&g1));
         MH_HIP(hipEventRecord(g0, streams().main));
         PassStats stats;
+        if (r.monitor) {
+            const mh_monitor_fn fn = r.monitor;
+            void* const data = r.monitor_data;
+            stats.monitor = [fn, data](uint64_t crossed) { return fn(data, crossed) != 0; };
+        }
         apply_blocks(state.get(), n, r.precision, p.blocks, window_for(r), r.record_pass_times, &stats,
                      r.max_pass_cost);
         MH_HIP(hipEventRecord(g1, streams().main));
         synchronize();
         float gate_ms = 0;
         MH_HIP(hipEventElapsedTime(&gate_ms, g0, g1));
-        (void)hipEventDestroy(g0);
-        (void)hipEventDestroy(g1);
         result.gate_seconds = gate_ms * 1e-3;
         result.num_passes = stats.passes;
         result.pass_ms = std::move(stats.pass_ms);
@@ -239,6 +250,8 @@ ExecutionRequest to_request(const mh_execution_request* q) {
     r.budget_bytes = q->budget_bytes;
     r.record_pass_times = q->record_pass_times != 0;
     r.max_pass_cost = q->max_pass_cost;
+    r.monitor = q->monitor;
+    r.monitor_data = q->monitor_data;
     return r;
 }
 
@@ -311,3 +324,28 @@ int mh_engine_execute(const mh_execution_request* request, mh_execution_result* 
 namespace mh {
 void set_calibration(const mh_calibration& c) { g_calibration = c; }
 }  // namespace mh
+
+extern "C" {
+
+int mh_set_calibration(const mh_calibration* profile) {
+    return guarded([&] {
+        if (profile == nullptr) fail(MH_ERR_INVALID, "null profile");
+        const double rates[] = {profile->hbm_bytes_per_second, profile->pass_bytes_per_second,
+                                profile->h2d_bytes_per_second, profile->d2h_bytes_per_second,
+                                profile->fp64_ops_per_second};
+        for (double r : rates)
+            if (!(r > 0.0) || !std::isfinite(r)) fail(MH_ERR_INVALID, "profile rates must be finite and positive");
+        if (!(profile->launch_seconds >= 0.0) || !std::isfinite(profile->launch_seconds))
+            fail(MH_ERR_INVALID, "launch_seconds must be finite and non-negative");
+        set_calibration(*profile);
+    });
+}
+
+int mh_get_calibration(mh_calibration* out) {
+    return guarded([&] {
+        if (out == nullptr) fail(MH_ERR_INVALID, "null output");
+        *out = g_calibration;
+    });
+}
+
+}  // extern "C"

@@ -66,6 +66,8 @@ struct ExecutionRequest {
     uint64_t budget_bytes = 0;
     bool record_pass_times = false;
     uint32_t max_pass_cost = 0;
+    mh_monitor_fn monitor = nullptr;
+    void* monitor_data = nullptr;
 };
 
 struct ExecutionResult {

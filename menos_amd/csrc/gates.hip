@@ -11,13 +11,14 @@
 #include <algorithm>
 #include <cstring>
 
+#include "gate_index.h"
 #include "mh_internal.h"
 #include "window.h"
 
 namespace mh {
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = kGroupThreads;
 constexpr uint32_t kMaxK = 6;
 
 // ---- complex arithmetic
@@ -56,20 +57,10 @@ __device__ __forceinline__ void generic_group(const GenericArgs& a, const typena
     using V = typename Vec<R>::type;
     constexpr uint32_t dim = 1u << K;
     V* s = static_cast<V*>(a.state);
-    uint64_t base = g;
-#pragma unroll
-    for (uint32_t j = 0; j < K; ++j) {
-        const uint64_t lowm = (1ull << a.sorted[j]) - 1ull;
-        base = ((base & ~lowm) << 1) | (base & lowm);
-    }
+    const uint64_t base = group_base(g, a.sorted, K);
     V v[dim];
 #pragma unroll
-    for (uint32_t l = 0; l < dim; ++l) {
-        uint64_t idx = base;
-#pragma unroll
-        for (uint32_t j = 0; j < K; ++j) idx |= ((l >> j) & 1u) ? (1ull << a.ops[j]) : 0ull;
-        v[l] = s[idx];
-    }
+    for (uint32_t l = 0; l < dim; ++l) v[l] = s[group_member(base, l, a.ops, K)];
 #pragma unroll(K <= 2 ? dim : 1)
     for (uint32_t o = 0; o < dim; ++o) {
         R sr = R(0), si = R(0);
@@ -78,25 +69,20 @@ __device__ __forceinline__ void generic_group(const GenericArgs& a, const typena
             const V e = m[o * dim + in];
             cmac(sr, si, e.x, e.y, v[in].x, v[in].y);
         }
-        uint64_t idx = base;
-#pragma unroll
-        for (uint32_t j = 0; j < K; ++j) idx |= ((o >> j) & 1u) ? (1ull << a.ops[j]) : 0ull;
         V r;
         r.x = sr;
         r.y = si;
-        s[idx] = r;
+        s[group_member(base, o, a.ops, K)] = r;
     }
 }
 
-// One thread per group of 2^K amplitudes; the matrix is staged in LDS.
+// One thread per group of 2^K amplitudes. The matrix stays in global memory:
+// every lane of a wave reads the same entry at the same time, so the entries
+// come in through scalar loads and the lanes' registers hold amplitudes only.
 template <typename R, uint32_t K>
 __global__ void __launch_bounds__(kThreads) generic_gate_kernel(GenericArgs a) {
     using V = typename Vec<R>::type;
-    extern __shared__ unsigned char lds_raw[];
-    V* m = reinterpret_cast<V*>(lds_raw);
-    const V* src = static_cast<const V*>(a.matrix);
-    for (uint32_t e = threadIdx.x; e < (1u << (2 * K)); e += kThreads) m[e] = src[e];
-    __syncthreads();
+    const V* m = static_cast<const V*>(a.matrix);
     const uint64_t stride = uint64_t(gridDim.x) * kThreads;
     for (uint64_t g = uint64_t(blockIdx.x) * kThreads + threadIdx.x; g < a.groups; g += stride)
         generic_group<R, K>(a, m, g);
@@ -115,10 +101,7 @@ __global__ void __launch_bounds__(kThreads) init_kernel(void* state, uint64_t si
     }
 }
 
-uint32_t grid_for(uint64_t items) {
-    const uint64_t blocks = (items + kThreads - 1) / kThreads;
-    return static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(blocks, 1), uint64_t{1} << 20));
-}
+uint32_t grid_for(uint64_t items) { return group_grid(items); }
 
 template <typename R>
 void run_generic(void* state, uint32_t n, const cplx* matrix, const uint32_t* operands, uint32_t k) {
@@ -135,19 +118,22 @@ void run_generic(void* state, uint32_t n, const cplx* matrix, const uint32_t* op
     GenericArgs a{};
     a.state = state;
     a.matrix = dm.get();
+    if (k < 1 || k > kMaxK || k > n) fail(MH_ERR_INTERNAL, "generic kernel: block width out of range");
     a.groups = (uint64_t{1} << n) >> k;
     a.k = k;
     for (uint32_t j = 0; j < k; ++j) a.ops[j] = a.sorted[j] = operands[j];
     std::sort(a.sorted, a.sorted + k);
-    const std::size_t lds = sizeof(V) << (2 * k);
+    for (uint32_t j = 0; j < k; ++j)
+        if (a.sorted[j] >= n || (j > 0 && a.sorted[j] == a.sorted[j - 1]))
+            fail(MH_ERR_INTERNAL, "generic kernel: operands out of range or repeated");
     const dim3 grid(grid_for(a.groups)), block(kThreads);
     switch (k) {
-        case 1: hipLaunchKernelGGL((generic_gate_kernel<R, 1>), grid, block, lds, streams().main, a); break;
-        case 2: hipLaunchKernelGGL((generic_gate_kernel<R, 2>), grid, block, lds, streams().main, a); break;
-        case 3: hipLaunchKernelGGL((generic_gate_kernel<R, 3>), grid, block, lds, streams().main, a); break;
-        case 4: hipLaunchKernelGGL((generic_gate_kernel<R, 4>), grid, block, lds, streams().main, a); break;
-        case 5: hipLaunchKernelGGL((generic_gate_kernel<R, 5>), grid, block, lds, streams().main, a); break;
-        default: hipLaunchKernelGGL((generic_gate_kernel<R, 6>), grid, block, lds, streams().main, a); break;
+        case 1: hipLaunchKernelGGL((generic_gate_kernel<R, 1>), grid, block, 0, streams().main, a); break;
+        case 2: hipLaunchKernelGGL((generic_gate_kernel<R, 2>), grid, block, 0, streams().main, a); break;
+        case 3: hipLaunchKernelGGL((generic_gate_kernel<R, 3>), grid, block, 0, streams().main, a); break;
+        case 4: hipLaunchKernelGGL((generic_gate_kernel<R, 4>), grid, block, 0, streams().main, a); break;
+        case 5: hipLaunchKernelGGL((generic_gate_kernel<R, 5>), grid, block, 0, streams().main, a); break;
+        default: hipLaunchKernelGGL((generic_gate_kernel<R, 6>), grid, block, 0, streams().main, a); break;
     }
     MH_HIP(hipGetLastError());
     synchronize();

@@ -188,8 +188,8 @@ FusionPlan finish_plan(const std::vector<uint32_t>& block_of, const std::vector<
     return p;
 }
 
-// Containment fusion, the reference's default (src/ir/fuse_gates.cpp:135-147,
-// 190-283): a gate joins the newest open block sharing a qubit with it only if
+// Containment fusion, the reference's default (plan_fusion,
+// src/ir/fuse_gates.cpp:108-201, with may_join at :53-65): a gate joins the newest open block sharing a qubit with it only if
 // one qubit set contains the other (diagonal rule aside); a measurement is a
 // barrier block that closes every open block; blocks run in opening order.
 FusionPlan containment_plan(const CircuitView& cv, bool keep_pairs) {

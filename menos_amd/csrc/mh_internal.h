@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <complex>
+#include <functional>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -130,6 +131,9 @@ struct Block {
 struct PassStats {
     uint32_t passes = 0;
     std::vector<float> pass_ms;  // filled when timing is requested
+    // Called after each completed pass with the passes completed so far;
+    // true abandons (the queued work is waited for, then MH_ERR_ABANDONED).
+    std::function<bool(uint64_t)> monitor;
 };
 
 // Applies the blocks in order, grouped into window passes.

@@ -15,6 +15,7 @@
 #include <memory>
 #include <string>
 #include <thread>
+#include <vector>
 
 #include "batch.h"
 #include "engine.hpp"
@@ -44,10 +45,19 @@ public:
 
     std::shared_ptr<void> lane_setup() override { return std::make_shared<OwnStreams>(); }
 
-    void* arena_resize(uint64_t bytes) override {
-        arena_.reset();
-        if (bytes) arena_ = DeviceBuffer(bytes);
-        return arena_.get();
+    void* segment_alloc(uint64_t bytes) override {
+        DeviceBuffer b(bytes);
+        void* p = b.get();
+        segments_.push_back(std::move(b));
+        return p;
+    }
+
+    void segment_free(void* segment) noexcept override {
+        for (auto it = segments_.begin(); it != segments_.end(); ++it)
+            if (it->get() == segment) {
+                segments_.erase(it);
+                return;
+            }
     }
 
     void plan(serve::Job& j, int32_t fusion, uint32_t width, std::shared_ptr<const void>& plan) override {
@@ -92,7 +102,7 @@ public:
     }
 
 private:
-    DeviceBuffer arena_;
+    std::vector<DeviceBuffer> segments_;  // touched under the server's lock only
 };
 
 }  // namespace

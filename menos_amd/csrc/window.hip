@@ -1136,6 +1136,28 @@ void launch_window_passes(void* state, uint32_t n, const std::vector<Block>& blo
         for (auto& e : events) MH_HIP(hipEventCreate(&e));
         MH_HIP(hipEventRecord(events[0], streams().main));
     }
+    // With a monitor: an event after each pass; the monitor for pass p runs
+    // once pass p + 1 is queued, so the device never waits for the host.
+    const bool monitored = stats && stats->monitor;
+    std::vector<hipEvent_t> done;
+    struct EventsGuard {
+        std::vector<hipEvent_t>& list;
+        ~EventsGuard() {
+            for (hipEvent_t e : list) (void)hipEventDestroy(e);
+        }
+    } done_guard{done}, time_guard{events};
+    if (monitored) {
+        done.resize(passes.size());
+        for (auto& e : done) MH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    auto boundary = [&](std::size_t p) {
+        MH_HIP(hipEventSynchronize(done[p]));
+        if (stats->monitor(p + 1)) {
+            synchronize();
+            fail(MH_ERR_ABANDONED, "the execution monitor abandoned the run after " + std::to_string(p + 1) +
+                                       " of " + std::to_string(passes.size()) + " passes");
+        }
+    };
     for (std::size_t p = 0; p < passes.size(); ++p) {
         if (passes[p].generic) {
             const Block& blk = blocks[passes[p].blocks[0]];
@@ -1159,13 +1181,17 @@ void launch_window_passes(void* state, uint32_t n, const std::vector<Block>& blo
             MH_HIP(hipGetLastError());
         }
         if (time_passes) MH_HIP(hipEventRecord(events[p + 1], streams().main));
+        if (monitored) {
+            MH_HIP(hipEventRecord(done[p], streams().main));
+            if (p > 0) boundary(p - 1);
+        }
     }
+    if (monitored && !passes.empty()) boundary(passes.size() - 1);
     synchronize();
     if (time_passes) {
         stats->pass_ms.resize(passes.size());
         for (std::size_t p = 0; p < passes.size(); ++p)
             MH_HIP(hipEventElapsedTime(&stats->pass_ms[p], events[p], events[p + 1]));
-        for (auto& e : events) (void)hipEventDestroy(e);
     }
 }
 

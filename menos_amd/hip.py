@@ -18,7 +18,10 @@ LIB_PATH = os.path.join(HERE, "libmenos_hip.so")
 
 C128, C64 = 0, 1
 FUSE_NONE, FUSE_CONTAINMENT, FUSE_WIDTH, FUSE_PAIRS = 0, 1, 2, 3
-MH_OK, MH_ERR_INVALID, MH_ERR_RESOURCE, MH_ERR_INTERNAL, MH_ERR_DEVICE = range(5)
+MH_OK, MH_ERR_INVALID, MH_ERR_RESOURCE, MH_ERR_INTERNAL, MH_ERR_DEVICE, MH_ERR_ABANDONED = range(6)
+
+# int (*)(void* user_data, uint64_t boundaries_crossed): nonzero abandons.
+MONITOR_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64)
 
 # Every entry point include/menos_hip.h declares.
 EXPORTS = (
@@ -28,7 +31,7 @@ EXPORTS = (
     "mh_copy_from_host", "mh_state_digest", "mh_engine_estimate", "mh_engine_execute", "mh_plan_fusion",
     "mh_gate_matrix", "mh_calibrate", "mh_batch_execute", "mh_state_max_abs_diff", "mh_server_create",
     "mh_server_submit", "mh_server_wait", "mh_server_replay", "mh_server_stats_get", "mh_server_destroy",
-    "mh_derive_seeds",
+    "mh_derive_seeds", "mh_set_calibration", "mh_get_calibration",
 )
 JOB_ESTIMATOR, JOB_SAMPLER = 0, 1
 
@@ -69,7 +72,8 @@ class CRequest(ctypes.Structure):
                 ("target_qubits", _u32p), ("target_clbits", _u32p), ("num_targets", ctypes.c_uint32),
                 ("bitstring_width", ctypes.c_uint32), ("num_terms", ctypes.c_uint32), ("x_masks", _u64p),
                 ("z_masks", _u64p), ("budget_bytes", ctypes.c_uint64), ("want_statevector", ctypes.c_int32),
-                ("record_pass_times", ctypes.c_int32), ("max_pass_cost", ctypes.c_uint32)]
+                ("record_pass_times", ctypes.c_int32), ("max_pass_cost", ctypes.c_uint32),
+                ("monitor", MONITOR_FN), ("monitor_data", ctypes.c_void_p)]
 
 
 class CResult(ctypes.Structure):
@@ -124,6 +128,13 @@ def device_info() -> DeviceInfo:
     info = DeviceInfo()
     check(lib().mh_device_query(ctypes.byref(info)))
     return info
+
+
+def available_bytes() -> int:
+    """Free device bytes plus what the pool holds for reuse."""
+    v = ctypes.c_uint64()
+    check(lib().mh_device_available_bytes(ctypes.byref(v)))
+    return v.value
 
 
 def amp_dtype(precision: int):
@@ -300,12 +311,26 @@ class HipDenseEngine:
 
     def execute(self, circuit: W.Circuit, *, precision=C128, fusion=FUSE_CONTAINMENT, fusion_width=0,
                 window_qubits=0, shots=0, seed=0, targets=None, width=None, terms=(), budget_bytes=0,
-                statevector=True, record_pass_times=False, max_passes=4096, max_pass_cost=0) -> Result:
+                statevector=True, record_pass_times=False, max_passes=4096, max_pass_cost=0,
+                monitor=None) -> Result:
+        """``monitor(boundaries_crossed)`` runs after every HBM pass; it
+        abandons the run by raising (as the reference's at_safe_boundary
+        does), which this call re-raises after the engine has freed the state."""
         keep = _CircuitKeep(circuit)
         req, hold = self._request(circuit, keep, precision=precision, fusion=fusion, fusion_width=fusion_width,
                                   window_qubits=window_qubits, shots=shots, seed=seed, targets=targets, width=width,
                                   terms=terms, budget_bytes=budget_bytes, statevector=statevector,
                                   record_pass_times=record_pass_times, max_pass_cost=max_pass_cost)
+        raised = []
+        if monitor is not None:
+            def _boundary(_data, crossed):
+                try:
+                    monitor(int(crossed))
+                    return 0
+                except BaseException as e:  # handed back to the caller below
+                    raised.append(e)
+                    return 1
+            req.monitor = MONITOR_FN(_boundary)
         sv = np.zeros(1 << circuit.num_qubits, amp_dtype(precision)) if statevector else None
         outc = np.zeros(max(shots, 1), np.uint64)
         cnt = np.zeros(max(shots, 1), np.uint64)
@@ -314,7 +339,10 @@ class HipDenseEngine:
         res = CResult(sv.ctypes.data_as(ctypes.c_void_p) if sv is not None else None,
                       outc.ctypes.data_as(_u64p), cnt.ctypes.data_as(_u64p), 0, ex.ctypes.data_as(_f64p),
                       pms.ctypes.data_as(_f32p), max_passes)
-        check(lib().mh_engine_execute(ctypes.byref(req), ctypes.byref(res)))
+        status = lib().mh_engine_execute(ctypes.byref(req), ctypes.byref(res))
+        if status == MH_ERR_ABANDONED and raised:
+            raise raised[0]
+        check(status)
         counts = {int(outc[i]): int(cnt[i]) for i in range(res.num_outcomes)} if shots else None
         return Result(statevector=sv, counts=counts,
                       expectation=ex.view(np.complex128)[:len(terms)].copy() if terms else None,
@@ -535,4 +563,14 @@ class Server:
 def calibrate() -> Calibration:
     c = Calibration()
     check(lib().mh_calibrate(ctypes.byref(c)))
+    return c
+
+
+def set_calibration(c: Calibration) -> None:
+    check(lib().mh_set_calibration(ctypes.byref(c)))
+
+
+def get_calibration() -> Calibration:
+    c = Calibration()
+    check(lib().mh_get_calibration(ctypes.byref(c)))
     return c

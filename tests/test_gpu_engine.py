@@ -56,14 +56,7 @@ def test_random_circuit_20q_20_layers(orc, hip, golden):
     assert np.max(np.abs(r.expectation - cpx(g["z_expectation"]))) <= 1e-12
 
 
-WIDE_WINDOW_SKIP = ("dense blocks of 3+ qubits in 12-qubit LDS windows: a GPU run of this case went silent; "
-                    "off the GPU until its cause is shown")
-
-
-@pytest.mark.parametrize("width", [2, pytest.param(3, marks=pytest.mark.skip(reason=WIDE_WINDOW_SKIP)),
-                                   pytest.param(4, marks=pytest.mark.skip(reason=WIDE_WINDOW_SKIP)),
-                                   pytest.param(5, marks=pytest.mark.skip(reason=WIDE_WINDOW_SKIP)),
-                                   pytest.param(6, marks=pytest.mark.skip(reason=WIDE_WINDOW_SKIP))])
+@pytest.mark.parametrize("width", [2, 3, 4, 5, 6])
 def test_width_fusion_matches_oracle_on_same_plan(orc, hip, width):
     c = W.random_circuit(14, 6, 40 + width)
     order, bb, bq, qb, _, _ = hip.plan_fusion(c, hip.FUSE_WIDTH, width)
@@ -102,7 +95,6 @@ def test_pair_fusion_matches_oracle_on_same_plan(orc, hip, kind):
         assert plan.num_blocks == 4 * n
 
 
-@pytest.mark.skip(reason=WIDE_WINDOW_SKIP)
 @pytest.mark.parametrize("seed", range(4))
 def test_mixed_circuits(orc, hip, seed):
     for c in (random_mixed_circuit(11, 90, 1000 + seed, measures=True), diagonal_heavy_circuit(13, 120, 1100 + seed)):
@@ -147,3 +139,64 @@ def test_peak_device_bytes_reported(hip):
     r = hip.HipDenseEngine().execute(W.random_circuit(18, 2, 1), shots=10, seed=1, targets=[(0, 0)], width=1)
     assert r.actual_bytes == 16 << 18
     assert r.peak_device_bytes >= r.actual_bytes
+
+
+def test_monitor_sees_every_pass_and_changes_nothing(orc, hip):
+    """The safe-boundary monitor runs once per pass, counting from 1, and a
+    monitored run is bitwise the unmonitored one."""
+    c = W.random_circuit(18, 8, 77)
+    seen = []
+    eng = hip.HipDenseEngine()
+    r = eng.execute(c, monitor=seen.append, targets=[], width=0)
+    plain = eng.execute(c, targets=[], width=0)
+    assert seen == list(range(1, r.num_passes + 1))
+    assert r.statevector.tobytes() == plain.statevector.tobytes()
+    want, _, _ = orc.execute(c)
+    assert r.statevector.tobytes() == want.tobytes()
+
+
+def test_monitor_abandon_frees_the_state(hip):
+    """A monitor that raises at its third boundary abandons the run: the call
+    re-raises it, the state goes back to the pool (available device bytes as
+    before) and the next run is unaffected (reference
+    tests/api/cuda_execution_monitor_test.cpp)."""
+    class Abandon(Exception):
+        pass
+
+    def stop_at_three(crossed):
+        if crossed == 3:
+            raise Abandon(crossed)
+
+    c = W.random_circuit(24, 8, 5)                 # 256 MiB of state
+    eng = hip.HipDenseEngine()
+    eng.execute(c, statevector=False, targets=[], width=0)   # the pool holds a state of this size
+    before = hip.available_bytes()
+    with pytest.raises(Abandon):
+        eng.execute(c, monitor=stop_at_three, statevector=False, targets=[], width=0)
+    assert hip.available_bytes() == before
+    small = W.random_circuit(12, 4, 6)
+    assert eng.execute(small, targets=[], width=0).statevector.tobytes() == \
+        eng.execute(small, targets=[], width=0).statevector.tobytes()
+
+
+def test_calibration_and_time_model(hip):
+    """mh_calibrate measures finite, positive rates of the right magnitude and
+    installs them; the time model then predicts a 26-qubit run's device work
+    within a factor of two of the measured time."""
+    cal = hip.calibrate()
+    for f in ("hbm_bytes_per_second", "pass_bytes_per_second", "h2d_bytes_per_second", "d2h_bytes_per_second",
+              "fp64_ops_per_second"):
+        v = getattr(cal, f)
+        assert np.isfinite(v) and v > 0, f
+    assert 1e12 < cal.hbm_bytes_per_second < 1.2e13
+    assert 1e12 < cal.pass_bytes_per_second < 1.2e13
+    assert 0 < cal.launch_seconds < 1e-3
+    got = hip.get_calibration()
+    assert got.hbm_bytes_per_second == cal.hbm_bytes_per_second
+    c = W.random_circuit(26, 8, 11)
+    eng = hip.HipDenseEngine()
+    est = eng.estimate(c)
+    eng.execute(c, statevector=False, targets=[], width=0)
+    r = eng.execute(c, statevector=False, targets=[], width=0)
+    assert est.passes == r.num_passes
+    assert 0.5 < est.seconds / r.execute_seconds < 2.0, (est.seconds, r.execute_seconds)

@@ -46,11 +46,6 @@ def test_every_ordered_two_qubit_pair(orc, hip):
                 assert got.tobytes() == want.tobytes(), (n, a, b)
 
 
-WIDE_WINDOW_SKIP = ("dense blocks of 3+ qubits in 12-qubit LDS windows: a GPU run of this case went silent; "
-                    "off the GPU until its cause is shown")
-
-
-@pytest.mark.skip(reason=WIDE_WINDOW_SKIP)
 @pytest.mark.parametrize("k", [3, 4, 5, 6])
 def test_wide_blocks_window_and_generic(orc, hip, k):
     rng = np.random.default_rng(2423 + k)
