@@ -41,15 +41,28 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(qubits: int, layers: int, threads: int) -> dict:
-    """The reference CPU engine (oracle/_ref, DenseEngine::execute) on a bounded
-    sample of the same workload, as GB/s of its own gate passes (one read and
-    one write of the state per plan block)."""
+def host_cores() -> int:
+    """Cores this process may run on (what `nproc` reports)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(qubits: int, layers: int, threads: int, gpu_passes: int, full_qubits: int,
+                 full_layers: int) -> dict:
+    """The reference CPU engine (oracle/_ref, DenseEngine::execute with the
+    reference's containment plan) on a bounded sample of the same workload:
+    the random circuit at `qubits` qubits and `layers` layers. `value` is in
+    the unit of the GPU figure: the GPU pass plan's algorithmic bytes for the
+    same sample circuit (gpu_passes x 2 x 16 x 2^qubits) over the CPU's
+    execute time, so value / cpu value is the speed-up on the same circuit.
+    The full workload's CPU time is extrapolated from the sample (state size
+    and layer count scale the time linearly; the sample's state is already far
+    beyond the host caches) and labelled as such."""
     import oracle
     from menos_amd import workloads as W
     c = W.random_circuit(qubits, layers, 2025)
-    plan = oracle.Oracle().plan_fusion(c)
-    blocks = plan.num_blocks
     if oracle.Reference.available():
         _, _, _, secs = oracle.Reference().execute(c, fuse_gates=True, statevector=False, threads=threads)
         kind = "reference"
@@ -58,10 +71,17 @@ def cpu_baseline(qubits: int, layers: int, threads: int) -> dict:
         oracle.Oracle().execute(c, fuse_gates=True, statevector=False, threads=threads)
         secs = time.perf_counter() - t0
         kind = "port"
-    algo = blocks * 2 * 16 * (1 << qubits)
+    algo = gpu_passes * 2 * 16 * (1 << qubits)
+    per_layer = secs / layers
+    full = per_layer * (1 << (full_qubits - qubits)) * full_layers
     return {"value": round(algo / secs / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": kind,
-            "sample": f"random circuit {qubits} qubits complex128, {layers} layers ({blocks} gate passes), "
-                      f"{secs:.2f} s on {threads} threads"}
+            "sample": f"random circuit {qubits} qubits complex128, {layers} layers, reference containment plan, "
+                      f"{secs:.2f} s on {threads} threads (nproc {host_cores()}); GB/s = the GPU pass plan's "
+                      f"{gpu_passes} passes x 2 x 16 x 2^{qubits} B over that time",
+            "seconds_per_layer": round(per_layer, 4),
+            "extrapolated_full_seconds": round(full, 1),
+            "extrapolated_full": f"{full_qubits} qubits, {full_layers} layers: {full:.0f} s (extrapolated from "
+                                 f"the {qubits}-qubit sample, not measured)"}
 
 
 def batched(args, H, W) -> dict:
@@ -245,8 +265,8 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=0.0, help="arrivals per second; 0: 0.8 x measured capacity")
     ap.add_argument("--serve-estimator-precision", choices=["c64", "c128"], default="c64")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--c64-report", action="store_true",
-                    help="also measure the complex64 form at --c64-qubits and its error against complex128")
+    ap.add_argument("--no-c64-report", action="store_true",
+                    help="skip the complex64 form at --c64-qubits and its error against complex128")
     ap.add_argument("--c64-qubits", type=int, default=34)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run the library's calibration (its streaming copy of 2 GiB each way is the "
@@ -343,12 +363,15 @@ def main() -> int:
         out["batched"] = batched(args, H, W)
     if not args.no_serve:
         out["served"] = served(args, H, W)
-    if args.c64_report:
+    if not args.no_c64_report:
         out["complex64"] = complex64_report(args, H, W)
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: {args.cpu_qubits} qubits, {args.cpu_layers} layers, {threads} threads")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_qubits, args.cpu_layers, threads)
+        threads = host_cores()
+        sample = W.random_circuit(args.cpu_qubits, args.cpu_layers, 2025)
+        gpu_passes = eng.estimate(sample, precision=H.C128).passes
+        log(f"cpu baseline: {args.cpu_qubits} qubits, {args.cpu_layers} layers, {threads} threads "
+            f"(GPU plan: {gpu_passes} passes)")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_qubits, args.cpu_layers, threads, gpu_passes, n, args.layers)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -102,13 +102,34 @@ int mh_apply_matrix_generic(void* state, uint32_t num_qubits, int32_t precision,
 /* A whole block list in one call: block b applies the matrix at
  * matrices + matrix_offsets[b] (in complex entries) to operands
  * operands + operand_offsets[b] .. operand_offsets[b+1]. Blocks are grouped into
- * LDS-window passes (many blocks per HBM read+write of the state); results are
+ * window passes (many blocks per HBM read+write of the state; each window
+ * sits in one wavefront's registers); results are
  * bitwise those of calling mh_apply_matrix block by block. window_qubits = 0
  * picks the default window. On return *passes (if non-null) holds the number
  * of HBM passes made. */
 int mh_apply_blocks(void* state, uint32_t num_qubits, int32_t precision, uint32_t num_blocks,
                     const uint32_t* operand_offsets, const uint32_t* operands, const uint64_t* matrix_offsets,
                     const double* matrices, uint32_t window_qubits, uint32_t* passes);
+
+/* The HBM passes mh_apply_blocks makes for a block list (the same arrays),
+ * without touching the device: pass p applies blocks
+ * [pass_begin[p], pass_begin[p+1]) (pass_begin has room for num_blocks + 1
+ * entries). Splitting a block list at these boundaries and applying each part
+ * with mh_apply_blocks gives the same state bit for bit. The reference's
+ * counterpart is the disk tier's pass plan (src/engines/dense_tiered/pass_plan.h:12-102),
+ * which groups blocks into windows the same way; its in-memory engines make
+ * one pass per block. */
+int mh_plan_passes(uint32_t num_qubits, int32_t precision, uint32_t num_blocks, const uint32_t* operand_offsets,
+                   const uint32_t* operands, const uint64_t* matrix_offsets, const double* matrices,
+                   uint32_t window_qubits, uint32_t max_pass_cost, uint32_t* pass_begin, uint32_t* num_passes);
+
+/* out[i] = state[indices[i]] for i < count (indices and out in host memory,
+ * amplitudes of the state's precision): a sample of a state too large to copy
+ * back, for the per-pass sampled checks of 33- and 34-qubit runs. No
+ * counterpart in the reference (its largest GPU states are copied back whole,
+ * DeviceBuffer::copy_to_host, device_buffer.h:25-56). */
+int mh_gather(const void* state, uint32_t num_qubits, int32_t precision, const uint64_t* indices, uint64_t count,
+              void* out);
 
 /* Raw expectation terms <psi|P_t|psi> (no coefficient), one per (x, z) mask
  * pair, out = 2 doubles per term. FP64 fixed-order reduction: reproducible

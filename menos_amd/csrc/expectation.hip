@@ -421,15 +421,11 @@ void expectation_terms(const void* state, uint32_t n, int32_t precision, const u
     expectation_batch(precision, {it}, reinterpret_cast<double*>(out));
 }
 
-// Every item's terms in two or three launches: X masks whose bits fit a tile
-// (with the lowest qubits) reduce from LDS tiles, a set of them per state read;
-// any other X mask (more than kTileBits - 3 bits) on the per-mask kernel; then
-// one finishing launch over every term.
-template <typename V>
-void expectation_batch_typed(const std::vector<ExpectationItem>& items, double* out) {
-    context();
-    uintptr_t base_addr = UINTPTR_MAX;
-    for (const ExpectationItem& it : items) base_addr = std::min(base_addr, reinterpret_cast<uintptr_t>(it.state));
+// The launches of one expectation call: the descriptors of every tile set,
+// per-mask job and term slot, and where each lands in one device workspace
+// (descriptors, block partials, output terms; 256-byte aligned regions).
+struct ExpPlan {
+    uintptr_t base_addr = UINTPTR_MAX;  // the lowest state address: kernels address states from it
     std::vector<ExpJob> jobs;
     std::vector<uint32_t> block_begin;
     std::vector<TileJob> tjobs;
@@ -438,6 +434,30 @@ void expectation_batch_typed(const std::vector<ExpectationItem>& items, double* 
     std::vector<ExpSlot> slots;
     uint64_t partial_doubles = 0;
     uint32_t total_blocks = 0, tile_blocks = 0;
+    uint32_t out_terms = 0;
+    // Workspace offsets in bytes.
+    uint64_t at_slots = 0, at_tjobs = 0, at_tterms = 0, at_tbegin = 0, at_jobs = 0, at_begin = 0;
+    uint64_t at_partials = 0, at_out = 0, bytes = 0;
+};
+
+// Plans every item's terms: X masks whose bits fit a tile (with the lowest
+// qubits) reduce from LDS tiles, a set of them per state read; any other X
+// mask (more than kTileBits - 3 bits) takes the per-mask kernel; then one
+// finishing launch runs over every term.
+template <typename V>
+ExpPlan plan_expectation(const std::vector<ExpectationItem>& items) {
+    ExpPlan P;
+    uintptr_t& base_addr = P.base_addr;
+    for (const ExpectationItem& it : items) base_addr = std::min(base_addr, reinterpret_cast<uintptr_t>(it.state));
+    std::vector<ExpJob>& jobs = P.jobs;
+    std::vector<uint32_t>& block_begin = P.block_begin;
+    std::vector<TileJob>& tjobs = P.tjobs;
+    std::vector<TileTerm>& tterms = P.tterms;
+    std::vector<uint32_t>& tblock_begin = P.tblock_begin;
+    std::vector<ExpSlot>& slots = P.slots;
+    uint64_t& partial_doubles = P.partial_doubles;
+    uint32_t& total_blocks = P.total_blocks;
+    uint32_t& tile_blocks = P.tile_blocks;
     auto add_slots = [&](uint64_t xm, const uint64_t* zs, const std::vector<uint32_t>& members, std::size_t at,
                          std::size_t cnt, uint32_t blocks, uint32_t out_index) {
         for (std::size_t k = 0; k < cnt; ++k) {
@@ -550,47 +570,76 @@ void expectation_batch_typed(const std::vector<ExpectationItem>& items, double* 
             }
         }
     }
-    if (slots.empty()) return;
-    uint32_t out_terms = 0;
-    for (const ExpSlot& sl : slots) out_terms = std::max(out_terms, sl.out + 1);
-    DeviceBuffer dpartials(partial_doubles * sizeof(double));
-    DeviceBuffer dslots(slots.size() * sizeof(ExpSlot)), dout(uint64_t(out_terms) * 2 * sizeof(double));
-    upload(dslots.get(), slots.data(), slots.size() * sizeof(ExpSlot));
-    DeviceBuffer dtjobs, dtgroups, dtbegin, djobs, dbegin;
-    if (!tjobs.empty()) {
-        dtjobs = DeviceBuffer(tjobs.size() * sizeof(TileJob));
-        dtgroups = DeviceBuffer(tterms.size() * sizeof(TileTerm));
-        dtbegin = DeviceBuffer(tblock_begin.size() * sizeof(uint32_t));
-        upload(dtjobs.get(), tjobs.data(), tjobs.size() * sizeof(TileJob));
-        upload(dtgroups.get(), tterms.data(), tterms.size() * sizeof(TileTerm));
-        upload(dtbegin.get(), tblock_begin.data(), tblock_begin.size() * sizeof(uint32_t));
-        hipLaunchKernelGGL(expectation_tile_kernel<V>, dim3(tile_blocks), dim3(kThreads), 0, streams().main,
-                           static_cast<const TileJob*>(dtjobs.as<TileJob>()),
-                           static_cast<const uint32_t*>(dtbegin.as<uint32_t>()), static_cast<uint32_t>(tjobs.size()),
-                           static_cast<const TileTerm*>(dtgroups.as<TileTerm>()), reinterpret_cast<const V*>(base_addr),
-                           dpartials.as<double>());
+    for (const ExpSlot& sl : slots) P.out_terms = std::max(P.out_terms, sl.out + 1);
+    uint64_t at = 0;
+    auto region = [&at](uint64_t bytes) {
+        const uint64_t here = at;
+        at += (bytes + 255) / 256 * 256;
+        return here;
+    };
+    P.at_slots = region(slots.size() * sizeof(ExpSlot));
+    P.at_tjobs = region(tjobs.size() * sizeof(TileJob));
+    P.at_tterms = region(tterms.size() * sizeof(TileTerm));
+    P.at_tbegin = region(tblock_begin.size() * sizeof(uint32_t));
+    P.at_jobs = region(jobs.size() * sizeof(ExpJob));
+    P.at_begin = region(block_begin.size() * sizeof(uint32_t));
+    P.at_partials = region(partial_doubles * sizeof(double));
+    P.at_out = region(uint64_t(P.out_terms) * 2 * sizeof(double));
+    P.bytes = at;
+    return P;
+}
+
+// Runs a plan with its workspace at `ws` (P.bytes bytes of device memory,
+// nothing in it need be zero: every partial and output is written before it is
+// read), then copies the terms to out[2 * term]. Uses the calling thread's main
+// stream and waits for it.
+template <typename V>
+void run_expectation(const ExpPlan& P, void* ws, double* out) {
+    if (P.slots.empty()) return;
+    char* w = static_cast<char*>(ws);
+    double* partials = reinterpret_cast<double*>(w + P.at_partials);
+    double* dout = reinterpret_cast<double*>(w + P.at_out);
+    upload(w + P.at_slots, P.slots.data(), P.slots.size() * sizeof(ExpSlot));
+    if (!P.tjobs.empty()) {
+        upload(w + P.at_tjobs, P.tjobs.data(), P.tjobs.size() * sizeof(TileJob));
+        upload(w + P.at_tterms, P.tterms.data(), P.tterms.size() * sizeof(TileTerm));
+        upload(w + P.at_tbegin, P.tblock_begin.data(), P.tblock_begin.size() * sizeof(uint32_t));
+        hipLaunchKernelGGL(expectation_tile_kernel<V>, dim3(P.tile_blocks), dim3(kThreads), 0, streams().main,
+                           reinterpret_cast<const TileJob*>(w + P.at_tjobs),
+                           reinterpret_cast<const uint32_t*>(w + P.at_tbegin), static_cast<uint32_t>(P.tjobs.size()),
+                           reinterpret_cast<const TileTerm*>(w + P.at_tterms), reinterpret_cast<const V*>(P.base_addr),
+                           partials);
         MH_HIP(hipGetLastError());
     }
-    if (!jobs.empty()) {
-        djobs = DeviceBuffer(jobs.size() * sizeof(ExpJob));
-        dbegin = DeviceBuffer(block_begin.size() * sizeof(uint32_t));
-        upload(djobs.get(), jobs.data(), jobs.size() * sizeof(ExpJob));
-        upload(dbegin.get(), block_begin.data(), block_begin.size() * sizeof(uint32_t));
-        hipLaunchKernelGGL(expectation_batch_kernel<V>, dim3(total_blocks), dim3(kThreads), 0, streams().main,
-                           static_cast<const ExpJob*>(djobs.as<ExpJob>()), static_cast<const uint32_t*>(dbegin.as<uint32_t>()),
-                           static_cast<uint32_t>(jobs.size()), reinterpret_cast<const V*>(base_addr), dpartials.as<double>());
+    if (!P.jobs.empty()) {
+        upload(w + P.at_jobs, P.jobs.data(), P.jobs.size() * sizeof(ExpJob));
+        upload(w + P.at_begin, P.block_begin.data(), P.block_begin.size() * sizeof(uint32_t));
+        hipLaunchKernelGGL(expectation_batch_kernel<V>, dim3(P.total_blocks), dim3(kThreads), 0, streams().main,
+                           reinterpret_cast<const ExpJob*>(w + P.at_jobs),
+                           reinterpret_cast<const uint32_t*>(w + P.at_begin), static_cast<uint32_t>(P.jobs.size()),
+                           reinterpret_cast<const V*>(P.base_addr), partials);
         MH_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(expectation_finish_batch_kernel, dim3(static_cast<uint32_t>(slots.size())), dim3(kThreads), 0,
-                       streams().main, static_cast<const ExpSlot*>(dslots.as<ExpSlot>()),
-                       static_cast<const double*>(dpartials.as<double>()), dout.as<double>());
+    hipLaunchKernelGGL(expectation_finish_batch_kernel, dim3(static_cast<uint32_t>(P.slots.size())), dim3(kThreads), 0,
+                       streams().main, reinterpret_cast<const ExpSlot*>(w + P.at_slots),
+                       static_cast<const double*>(partials), dout);
     MH_HIP(hipGetLastError());
-    std::vector<double> host(uint64_t(out_terms) * 2);
-    download(host.data(), dout.get(), host.size() * sizeof(double));
-    for (const ExpSlot& sl : slots) {
+    std::vector<double> host(uint64_t(P.out_terms) * 2);
+    download(host.data(), dout, host.size() * sizeof(double));
+    for (const ExpSlot& sl : P.slots) {
         out[2 * uint64_t(sl.out)] = host[2 * uint64_t(sl.out)];
         out[2 * uint64_t(sl.out) + 1] = host[2 * uint64_t(sl.out) + 1];
     }
+}
+
+// Every item's terms, with the workspace from the pool.
+template <typename V>
+void expectation_batch_typed(const std::vector<ExpectationItem>& items, double* out) {
+    context();
+    const ExpPlan P = plan_expectation<V>(items);
+    if (P.slots.empty()) return;
+    DeviceBuffer ws(P.bytes);
+    run_expectation<V>(P, ws.get(), out);
 }
 
 void expectation_batch(int32_t precision, const std::vector<ExpectationItem>& items, double* out) {

@@ -101,6 +101,14 @@ __global__ void __launch_bounds__(kThreads) init_kernel(void* state, uint64_t si
     }
 }
 
+// out[i] = state[index[i]] for i < count (indices checked on the host).
+template <typename V>
+__global__ void __launch_bounds__(kThreads) gather_kernel(const V* state, const uint64_t* index, uint64_t count,
+                                                         V* out) {
+    const uint64_t stride = uint64_t(gridDim.x) * kThreads;
+    for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < count; i += stride) out[i] = state[index[i]];
+}
+
 uint32_t grid_for(uint64_t items) { return group_grid(items); }
 
 template <typename R>
@@ -137,6 +145,28 @@ void run_generic(void* state, uint32_t n, const cplx* matrix, const uint32_t* op
     }
     MH_HIP(hipGetLastError());
     synchronize();
+}
+
+// Blocks from the C ABI's flat arrays (block b: operands
+// [operand_offsets[b], operand_offsets[b + 1]), its matrix at
+// matrix_offsets[b] complex entries).
+std::vector<Block> blocks_from_arrays(uint32_t num_blocks, const uint32_t* operand_offsets, const uint32_t* operands,
+                                      const uint64_t* matrix_offsets, const double* matrices) {
+    if (num_blocks > 0 && (operand_offsets == nullptr || operands == nullptr || matrix_offsets == nullptr ||
+                           matrices == nullptr))
+        fail(MH_ERR_INVALID, "null block arrays");
+    std::vector<Block> blocks(num_blocks);
+    const cplx* mats = reinterpret_cast<const cplx*>(matrices);
+    for (uint32_t b = 0; b < num_blocks; ++b) {
+        Block& blk = blocks[b];
+        if (operand_offsets[b + 1] < operand_offsets[b] || operand_offsets[b + 1] - operand_offsets[b] > kMaxK)
+            fail(MH_ERR_INVALID, "block " + std::to_string(b) + " has more than 6 operands");
+        blk.operands.assign(operands + operand_offsets[b], operands + operand_offsets[b + 1]);
+        const std::size_t dim = std::size_t{1} << blk.operands.size();
+        blk.matrix.assign(mats + matrix_offsets[b], mats + matrix_offsets[b] + dim * dim);
+        blk.diagonal = matrix_is_diagonal(blk.matrix.data(), dim);
+    }
+    return blocks;
 }
 
 void check_operands(uint32_t n, const uint32_t* operands, uint32_t k) {
@@ -249,18 +279,59 @@ int mh_apply_blocks(void* state, uint32_t num_qubits, int32_t precision, uint32_
     return guarded([&] {
         amp_bytes(precision);
         if (num_qubits > 40) fail(MH_ERR_INVALID, "num_qubits above 40");
-        std::vector<Block> blocks(num_blocks);
-        const cplx* mats = reinterpret_cast<const cplx*>(matrices);
-        for (uint32_t b = 0; b < num_blocks; ++b) {
-            Block& blk = blocks[b];
-            blk.operands.assign(operands + operand_offsets[b], operands + operand_offsets[b + 1]);
-            const std::size_t dim = std::size_t{1} << blk.operands.size();
-            blk.matrix.assign(mats + matrix_offsets[b], mats + matrix_offsets[b] + dim * dim);
-            blk.diagonal = matrix_is_diagonal(blk.matrix.data(), dim);
-        }
+        const std::vector<Block> blocks =
+            blocks_from_arrays(num_blocks, operand_offsets, operands, matrix_offsets, matrices);
         PassStats stats;
         apply_blocks(state, num_qubits, precision, blocks, window_qubits, false, &stats, 0);
         if (passes) *passes = stats.passes;
+    });
+}
+
+int mh_plan_passes(uint32_t num_qubits, int32_t precision, uint32_t num_blocks, const uint32_t* operand_offsets,
+                   const uint32_t* operands, const uint64_t* matrix_offsets, const double* matrices,
+                   uint32_t window_qubits, uint32_t max_pass_cost, uint32_t* pass_begin, uint32_t* num_passes) {
+    return guarded([&] {
+        amp_bytes(precision);
+        if (num_qubits > 40) fail(MH_ERR_INVALID, "num_qubits above 40");
+        if (pass_begin == nullptr || num_passes == nullptr) fail(MH_ERR_INVALID, "null output");
+        const std::vector<Block> blocks =
+            blocks_from_arrays(num_blocks, operand_offsets, operands, matrix_offsets, matrices);
+        for (const Block& b : blocks) check_operands(num_qubits, b.operands.data(), static_cast<uint32_t>(b.operands.size()));
+        const uint32_t wmax = window_width(precision);
+        const uint32_t w = window_qubits == 0 ? wmax : window_qubits;
+        if (w > wmax || w < 7) fail(MH_ERR_INVALID, "window_qubits must be 7.." + std::to_string(wmax));
+        const std::vector<WindowPass> passes =
+            plan_window_passes(num_qubits, precision, blocks, w, max_pass_cost == 0 ? kDefaultPassCost : max_pass_cost);
+        // Pass p covers blocks [pass_begin[p], pass_begin[p + 1]); a block the
+        // planner skips (no operands) stays in the pass before it.
+        for (std::size_t p = 0; p < passes.size(); ++p) pass_begin[p] = p == 0 ? 0 : passes[p].blocks.front();
+        pass_begin[passes.size()] = num_blocks;
+        *num_passes = static_cast<uint32_t>(passes.size());
+    });
+}
+
+int mh_gather(const void* state, uint32_t num_qubits, int32_t precision, const uint64_t* indices, uint64_t count,
+              void* out) {
+    return guarded([&] {
+        const std::size_t amp = amp_bytes(precision);
+        if (num_qubits > 40) fail(MH_ERR_INVALID, "num_qubits above 40");
+        if (count == 0) return;
+        if (state == nullptr || indices == nullptr || out == nullptr) fail(MH_ERR_INVALID, "null argument");
+        const uint64_t size = uint64_t{1} << num_qubits;
+        for (uint64_t i = 0; i < count; ++i)
+            if (indices[i] >= size) fail(MH_ERR_INVALID, "gather index " + std::to_string(indices[i]) + " beyond the state");
+        context();
+        DeviceBuffer didx(count * sizeof(uint64_t)), dout(count * amp);
+        upload(didx.get(), indices, count * sizeof(uint64_t));
+        const dim3 grid(grid_for(count)), block(kThreads);
+        if (precision == MH_C128)
+            hipLaunchKernelGGL(gather_kernel<double2>, grid, block, 0, streams().main, static_cast<const double2*>(state),
+                               didx.as<uint64_t>(), count, dout.as<double2>());
+        else
+            hipLaunchKernelGGL(gather_kernel<float2>, grid, block, 0, streams().main, static_cast<const float2*>(state),
+                               didx.as<uint64_t>(), count, dout.as<float2>());
+        MH_HIP(hipGetLastError());
+        download(out, dout.get(), count * amp);
     });
 }
 

@@ -32,6 +32,7 @@ EXPORTS = (
     "mh_gate_matrix", "mh_calibrate", "mh_batch_execute", "mh_state_max_abs_diff", "mh_server_create",
     "mh_server_submit", "mh_server_wait", "mh_server_replay", "mh_server_stats_get", "mh_server_destroy",
     "mh_derive_seeds", "mh_set_calibration", "mh_get_calibration",
+    "mh_plan_passes", "mh_gather",
 )
 JOB_ESTIMATOR, JOB_SAMPLER = 0, 1
 
@@ -195,24 +196,20 @@ class DeviceState:
 
     def apply_blocks(self, blocks, window_qubits=0) -> int:
         """blocks: [(matrix, operands)]. Returns the HBM pass count."""
-        offs, ops, moffs, mats = [0], [], [], []
-        at = 0
-        for m, q in blocks:
-            ops.extend(q)
-            offs.append(len(ops))
-            moffs.append(at)
-            flat = np.ascontiguousarray(m, np.complex128).reshape(-1)
-            mats.append(flat)
-            at += flat.size
-        offs = np.array(offs, np.uint32)
-        ops = np.array(ops, np.uint32)
-        moffs = np.array(moffs, np.uint64)
-        mats = np.concatenate(mats) if mats else np.zeros(1, np.complex128)
+        offs, ops, moffs, mats = _block_arrays(blocks)
         passes = ctypes.c_uint32()
         check(lib().mh_apply_blocks(self.ptr, self.n, self.precision, len(blocks), _ptr(offs, _u32p),
                                     _ptr(ops, _u32p), _ptr(moffs, _u64p), mats.ctypes.data_as(_f64p),
                                     window_qubits, ctypes.byref(passes)))
         return passes.value
+
+    def gather(self, indices) -> np.ndarray:
+        """The amplitudes at `indices` (host array of the state's dtype)."""
+        idx = np.ascontiguousarray(indices, np.uint64)
+        out = np.empty(max(idx.size, 1), amp_dtype(self.precision))
+        check(lib().mh_gather(self.ptr, self.n, self.precision, _ptr(idx, _u64p), ctypes.c_uint64(idx.size),
+                              out.ctypes.data_as(ctypes.c_void_p)))
+        return out[:idx.size]
 
     def expectation(self, terms) -> np.ndarray:
         xs = np.array([t[0] for t in terms], np.uint64)
@@ -235,6 +232,33 @@ class DeviceState:
         check(lib().mh_sample_unsorted(self.ptr, self.n, self.precision, _ptr(u, _f64p), ctypes.c_uint64(u.size),
                                        out.ctypes.data_as(_u64p)))
         return out[:u.size]
+
+
+def _block_arrays(blocks):
+    """The C ABI's flat block arrays: operand offsets, operands, matrix
+    offsets (complex entries), matrices."""
+    offs, ops, moffs, mats = [0], [], [], []
+    at = 0
+    for m, q in blocks:
+        ops.extend(q)
+        offs.append(len(ops))
+        moffs.append(at)
+        flat = np.ascontiguousarray(m, np.complex128).reshape(-1)
+        mats.append(flat)
+        at += flat.size
+    return (np.array(offs, np.uint32), np.array(ops, np.uint32), np.array(moffs, np.uint64),
+            np.concatenate(mats) if mats else np.zeros(1, np.complex128))
+
+
+def plan_passes(num_qubits: int, blocks, precision=C128, window_qubits=0, max_pass_cost=0) -> list[tuple[int, int]]:
+    """The HBM passes mh_apply_blocks makes: [(first block, end block)]."""
+    offs, ops, moffs, mats = _block_arrays(blocks)
+    begin = np.zeros(len(blocks) + 1, np.uint32)
+    count = ctypes.c_uint32()
+    check(lib().mh_plan_passes(num_qubits, precision, len(blocks), _ptr(offs, _u32p), _ptr(ops, _u32p),
+                               _ptr(moffs, _u64p), mats.ctypes.data_as(_f64p), window_qubits, max_pass_cost,
+                               _ptr(begin, _u32p), ctypes.byref(count)))
+    return [(int(begin[p]), int(begin[p + 1])) for p in range(count.value)]
 
 
 def gate_matrix(opcode: int, params=()) -> np.ndarray:
