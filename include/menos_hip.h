@@ -138,6 +138,19 @@ int mh_gather(const void* state, uint32_t num_qubits, int32_t precision, const u
 int mh_expectation(const void* state, uint32_t num_qubits, int32_t precision, const uint64_t* x_masks,
                    const uint64_t* z_masks, uint32_t num_terms, double* out);
 
+/* mh_expectation without any device allocation: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * same kernels and the same terms bit for bit, with their descriptors, block
+ * partials and outputs in a static device array of 32 MiB per device, held
+ * until the process ends. Term sets that need more are refused with
+ * MH_ERR_INVALID naming the bytes (a tile-reduced term takes 1 MiB of
+ * partials at 30 qubits, 256 KiB at 28, 64 KiB at 26 and below). Every call on one device shares that array, so calls that overlap on
+ * one device (from several host threads) race; the array keeps the last
+ * call's contents and nothing in it needs to start at zero. */
+int mh_expectation_noalloc(const void* state, uint32_t num_qubits, int32_t precision, const uint64_t* x_masks,
+                           const uint64_t* z_masks, uint32_t num_terms, double* out);
+
 /* The basis index every sorted uniform lands on under the reference's
  * sequential cumulative walk (reference sample.cpp:55-69): the first basis
  * whose running FP64 sum of |a|^2 exceeds u, or the last basis if none does.

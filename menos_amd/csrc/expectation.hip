@@ -23,6 +23,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <mutex>
 
 #include "mh_internal.h"
 
@@ -690,6 +691,49 @@ double max_diff(const void* a, const void* b, uint64_t size) {
 
 }  // namespace mh
 
+namespace mh {
+namespace {
+
+// The workspace of mh_expectation_noalloc: one array per device, held for the
+// process's lifetime and shared by every call on that device (each call
+// overwrites it; none needs it zeroed). Host code never passes the array's
+// name to a kernel or a copy (there it is a host-side stand-in): it uses the
+// device address hipGetSymbolAddress gives for the current device.
+constexpr uint64_t kStaticWorkspaceBytes = uint64_t{32} << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticWorkspaceBytes];
+
+// The device address of `scratch` on the current device, looked up on the
+// first call there (with that device current) and kept per device.
+void* static_workspace() {
+    context();
+    int device = 0;
+    MH_HIP(hipGetDevice(&device));
+    static std::mutex mu;
+    static std::map<int, void*> per_device;
+    std::lock_guard<std::mutex> lock(mu);
+    const auto it = per_device.find(device);
+    if (it != per_device.end()) return it->second;
+    void* ptr = nullptr;
+    MH_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (ptr == nullptr) fail(MH_ERR_INTERNAL, "hipGetSymbolAddress gave no address for the static workspace");
+    per_device.emplace(device, ptr);
+    return ptr;
+}
+
+template <typename V>
+void expectation_static(const ExpectationItem& item, double* out) {
+    const ExpPlan P = plan_expectation<V>({item});
+    if (P.slots.empty()) return;
+    if (P.bytes > kStaticWorkspaceBytes)
+        fail(MH_ERR_INVALID, "mh_expectation_noalloc: these terms need " + std::to_string(P.bytes) +
+                                 " bytes of workspace; the static workspace holds " +
+                                 std::to_string(kStaticWorkspaceBytes) + " (use mh_expectation)");
+    run_expectation<V>(P, static_workspace(), out);
+}
+
+}  // namespace
+}  // namespace mh
+
 using namespace mh;
 
 extern "C" int mh_expectation(const void* state, uint32_t num_qubits, int32_t precision, const uint64_t* x_masks,
@@ -715,5 +759,25 @@ extern "C" int mh_state_max_abs_diff(const void* a, int32_t precision_a, const v
             *out = max_diff<float2, double2>(a, b, size);
         else
             *out = max_diff<float2, float2>(a, b, size);
+    });
+}
+
+extern "C" int mh_expectation_noalloc(const void* state, uint32_t num_qubits, int32_t precision,
+                                      const uint64_t* x_masks, const uint64_t* z_masks, uint32_t num_terms,
+                                      double* out) {
+    return guarded([&] {
+        amp_bytes(precision);
+        if (num_terms > 0 && (x_masks == nullptr || z_masks == nullptr || out == nullptr))
+            fail(MH_ERR_INVALID, "mh_expectation_noalloc: null term masks or output");
+        ExpectationItem it;
+        it.state = state;
+        it.n = num_qubits;
+        it.terms = num_terms;
+        it.x = x_masks;
+        it.z = z_masks;
+        if (precision == MH_C128)
+            expectation_static<double2>(it, out);
+        else
+            expectation_static<float2>(it, out);
     });
 }

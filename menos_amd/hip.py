@@ -31,7 +31,7 @@ EXPORTS = (
     "mh_copy_from_host", "mh_state_digest", "mh_engine_estimate", "mh_engine_execute", "mh_plan_fusion",
     "mh_gate_matrix", "mh_calibrate", "mh_batch_execute", "mh_state_max_abs_diff", "mh_server_create",
     "mh_server_submit", "mh_server_wait", "mh_server_replay", "mh_server_stats_get", "mh_server_destroy",
-    "mh_derive_seeds", "mh_set_calibration", "mh_get_calibration",
+    "mh_derive_seeds", "mh_set_calibration", "mh_get_calibration", "mh_expectation_noalloc",
     "mh_plan_passes", "mh_gather",
 )
 JOB_ESTIMATOR, JOB_SAMPLER = 0, 1
@@ -211,12 +211,15 @@ class DeviceState:
                               out.ctypes.data_as(ctypes.c_void_p)))
         return out[:idx.size]
 
-    def expectation(self, terms) -> np.ndarray:
+    def expectation(self, terms, noalloc=False) -> np.ndarray:
+        """Raw terms; noalloc=True takes mh_expectation_noalloc (its static
+        device workspace, no allocation)."""
         xs = np.array([t[0] for t in terms], np.uint64)
         zs = np.array([t[1] for t in terms], np.uint64)
         out = np.zeros(2 * max(len(terms), 1), np.float64)
-        check(lib().mh_expectation(self.ptr, self.n, self.precision, _ptr(xs, _u64p), _ptr(zs, _u64p), len(terms),
-                                   out.ctypes.data_as(_f64p)))
+        fn = lib().mh_expectation_noalloc if noalloc else lib().mh_expectation
+        check(fn(self.ptr, self.n, self.precision, _ptr(xs, _u64p), _ptr(zs, _u64p), len(terms),
+                 out.ctypes.data_as(_f64p)))
         return out.view(np.complex128)[:len(terms)].copy()
 
     def sample(self, sorted_uniforms) -> np.ndarray:
