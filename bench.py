@@ -129,6 +129,59 @@ def batched(args, H, W) -> dict:
     out["speedup_c64"] = round(out["c64"]["circuits_per_s"] / per["c64"], 2)
     out["speedup_c128"] = round(out["c128"]["circuits_per_s"] / per["c128"], 2)
     log(f"per-circuit path on {k} circuits: {out['per_circuit_path']}; speedup c64 {out['speedup_c64']}x")
+    out["graph"] = batched_graph(args, H, circuits, terms, fusion, per)
+    return out
+
+
+def batched_graph(args, H, circuits, terms, fusion, per) -> dict:
+    """The variational loop on the resident, graph-captured batch
+    (mh_batch_graph_*): the same circuits with new angles every iteration.
+    An iteration is update (block matrices and pass programs rebuilt on host
+    threads, the matrix table staged) plus run (one graph launch: matrix upload,
+    initialisation, every pass, the term reductions, the terms back). Rates are
+    over `args.graph_iterations` iterations after one untimed one; the last
+    iteration's terms are checked against mh_batch_execute on the same angles
+    (bit for bit) and complex64 against complex128 (largest |difference|)."""
+    import numpy as np
+    sched = dict(group_bytes=args.group_bytes, kernel_streams=args.kernel_streams, cached_states=args.cached_states)
+    out = {"iterations": args.graph_iterations, "schedule": sched}
+    last = {}
+    for name, prec in (("c64", H.C64), ("c128", H.C128)):
+        rng = np.random.default_rng(7)
+        with H.BatchGraph(circuits, terms, precision=prec, fusion=fusion[prec], **sched) as g:
+            g.run()
+            upd_s = run_s = 0.0
+            lat = []
+            for it in range(args.graph_iterations + 1):
+                new = [c.with_angles(rng.uniform(-np.pi, np.pi, c.num_angles)) for c in circuits]
+                u0 = time.perf_counter()
+                g.update(new)
+                u1 = time.perf_counter()
+                e, res = g.run()
+                u2 = time.perf_counter()
+                if it == 0:
+                    continue
+                upd_s += u1 - u0
+                run_s += u2 - u1
+                lat.append((u2 - u0) * 1e3)
+            check = H.batch_execute(new, terms, precision=prec, fusion=fusion[prec], **sched).expectation
+            same = all(a.tobytes() == b.tobytes() for a, b in zip(e, check))
+            last[name] = e
+            it_n = args.graph_iterations
+            rate = it_n * len(circuits) / (upd_s + run_s)
+            out[name] = {"circuits_per_s": round(rate, 1),
+                         "replay_circuits_per_s": round(it_n * len(circuits) / run_s, 1),
+                         "update_ms": round(upd_s / it_n * 1e3, 2), "run_ms": round(run_s / it_n * 1e3, 2),
+                         "host_plan_ms": round(res.plan_seconds * 1e3, 2),
+                         "host_compile_ms": round(res.compile_seconds * 1e3, 2),
+                         "graph_launches": res.launches,
+                         "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                         "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                         "equals_batch_execute": same,
+                         "speedup_vs_per_circuit": round(rate / per[name], 2)}
+        log(f"graph {name}: {out[name]}")
+    out["max_abs_err_c64_vs_c128"] = float(max(np.max(np.abs(a - b)) if a.size else 0.0
+                                               for a, b in zip(last["c64"], last["c128"])))
     return out
 
 
@@ -259,6 +312,10 @@ def main() -> int:
     ap.add_argument("--batch-circuits", type=int, default=4096)
     ap.add_argument("--per-circuit-sample", type=int, default=256)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--graph-iterations", type=int, default=5)
+    ap.add_argument("--group-bytes", type=int, default=0, help="batch cache-group size in bytes (0: one launch per pass)")
+    ap.add_argument("--kernel-streams", type=int, default=0)
+    ap.add_argument("--cached-states", action="store_true")
     ap.add_argument("--only", choices=["batched", "served", "complex64"],
                     help="run one section alone and print its JSON object (for profiling)")
     ap.add_argument("--serve-jobs", type=int, default=1000)

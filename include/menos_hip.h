@@ -348,6 +348,14 @@ typedef struct mh_batch_request {
     uint32_t host_threads;         /* planning threads; 0: the hardware's, at most 16 */
     int32_t want_statevectors;     /* copy every final state into result->statevectors */
     uint32_t max_pass_cost;        /* FP operations per amplitude one HBM pass may carry; 0 = default */
+    /* The device schedule. group_bytes = 0: pass p of every circuit of a planning group in one launch.
+     * group_bytes > 0: circuits split, largest first, into cache groups of at most group_bytes of state,
+     * each running all its passes back to back (so its states stay in the 256 MB Infinity Cache), the
+     * groups rotating over kernel_streams streams (0: 2; at most 4). cached_states: ordinary (cached)
+     * state loads and stores instead of non-temporal ones. Results do not depend on any of these. */
+    uint64_t group_bytes;
+    uint32_t kernel_streams;
+    int32_t cached_states;
 } mh_batch_request;
 
 typedef struct mh_batch_result {
@@ -364,6 +372,37 @@ typedef struct mh_batch_result {
 } mh_batch_result;
 
 int mh_batch_execute(const mh_batch_request* request, mh_batch_result* result);
+
+/* ---- A resident batch replayed from a captured HIP graph: the variational
+ * estimator's loop (the same circuits with new parameters, again and again).
+ * Replaces re-running run_batch_with_options (src/api/batch.cpp:10-27) with
+ * the plan cache's parameter-only reuse (plan_job_cached /
+ * reparameterize_direct_plan, src/plan/plan_cache.cpp:233-276).
+ *
+ * mh_batch_graph_create plans and compiles every circuit of the request once,
+ * keeps every state resident in one arena (all of them must fit the device:
+ * MH_ERR_RESOURCE otherwise) with every table the kernels read, and captures
+ * the whole round (matrix upload from pinned memory, initialisation, every
+ * pass, the term reductions, the terms back to pinned memory) into one graph.
+ * Uses the request's circuits, precision, fusion, global phases, terms, host
+ * threads, pass cost and schedule fields; want_statevectors must be 0, and
+ * blocks must be at most 2-qubit dense or diagonal.
+ * mh_batch_graph_update takes new parameters for the same circuits (same
+ * opcodes, qubits and parameter counts) and global phases (null: 0): block
+ * matrices and pass programs are rebuilt on host threads and the new matrix
+ * table is staged for the next run. A circuit whose structure differs, or
+ * whose new matrices change its pass program (an entry that becomes or stops
+ * being exactly 0 or +-1), is refused with MH_ERR_INVALID and nothing changes.
+ * mh_batch_graph_run replays the graph once and writes the terms (2 doubles
+ * per term, term order) into result->expectation_terms; it fills rounds,
+ * launches (those the graph holds), the last update's plan and compile
+ * seconds, and total_seconds. Terms equal mh_batch_execute's bit for bit for
+ * the same circuits and schedule. One thread at a time per graph. */
+typedef struct mh_batch_graph mh_batch_graph;
+int mh_batch_graph_create(const mh_batch_request* request, mh_batch_graph** out);
+int mh_batch_graph_update(mh_batch_graph* graph, const mh_circuit* circuits, const double* global_phases);
+int mh_batch_graph_run(mh_batch_graph* graph, mh_batch_result* result);
+int mh_batch_graph_destroy(mh_batch_graph* graph);
 
 /* ---- Serving: a resident job server. Replaces the per-pub sequential
  * `run` loops of the reference's Python primitives (MENOSEstimator._run_pub,

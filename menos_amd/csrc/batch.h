@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "mh_internal.h"
+#include "window.h"
 
 namespace mh {
 
@@ -41,8 +42,15 @@ struct RoundStats {
     double sample_seconds = 0;
 };
 
-// Device bytes one state of a round takes in the round's arena.
+// Device bytes one state of a round takes in the round's arena (a multiple of 256).
 uint64_t round_state_bytes(uint32_t n, int32_t precision);
+
+// Initialises a round's arena on the main stream with kernels only (no memset,
+// no copy: capturable): every byte zero, then state i's first amplitude
+// (at amplitude offsets[i] from the arena) to (phases[2i], phases[2i+1]).
+// offsets and phases are device memory; bytes is a multiple of 16.
+void launch_round_init(int32_t precision, void* arena, uint64_t bytes, const uint64_t* offsets, const double* phases,
+                       uint32_t count);
 
 // Runs one round: every item's state in one pooled arena, initialised to its
 // global phase, pass p of every item in one launch, all expectation terms in
@@ -55,7 +63,8 @@ uint64_t round_state_bytes(uint32_t n, int32_t precision);
 // their blocks), while the device runs the groups before them.
 void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t threads, double* terms_out,
                    RoundStats* stats, void* arena = nullptr,
-                   const std::function<void(std::size_t, std::size_t)>& prepare = nullptr);
+                   const std::function<void(std::size_t, std::size_t)>& prepare = nullptr,
+                   const BatchSchedule& schedule = {});
 
 // Runs body(i) for i in [0, count) on up to `threads` host threads; rethrows
 // the first exception.

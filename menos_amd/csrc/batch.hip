@@ -46,6 +46,13 @@ __global__ void __launch_bounds__(kThreads) first_amplitude_kernel(V* base, cons
     base[offsets[i]] = v;
 }
 
+// Zeros `words` 16-byte words (a fill kernel rather than a memset node, so
+// that it can sit in a captured graph).
+__global__ void __launch_bounds__(kThreads) zero_kernel(uint4* p, uint64_t words) {
+    const uint64_t stride = uint64_t(gridDim.x) * kThreads;
+    for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < words; i += stride) p[i] = make_uint4(0, 0, 0, 0);
+}
+
 struct Planned {
     uint32_t n = 0;
     std::vector<Block> blocks;
@@ -75,12 +82,29 @@ void parallel_for(std::size_t count, uint32_t threads, const std::function<void(
     if (error) std::rethrow_exception(error);
 }
 
+void launch_round_init(int32_t precision, void* arena, uint64_t bytes, const uint64_t* offsets, const double* phases,
+                       uint32_t count) {
+    const uint64_t words = bytes / 16;
+    const uint32_t zgrid = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(1, (words + kThreads - 1) / kThreads), 8192));
+    hipLaunchKernelGGL(zero_kernel, dim3(zgrid), dim3(kThreads), 0, streams().main, static_cast<uint4*>(arena), words);
+    MH_HIP(hipGetLastError());
+    const dim3 grid((count + kThreads - 1) / kThreads), block(kThreads);
+    if (precision == MH_C128)
+        hipLaunchKernelGGL(first_amplitude_kernel<double2>, grid, block, 0, streams().main, static_cast<double2*>(arena),
+                           offsets, phases, count);
+    else
+        hipLaunchKernelGGL(first_amplitude_kernel<float2>, grid, block, 0, streams().main, static_cast<float2*>(arena),
+                           offsets, phases, count);
+    MH_HIP(hipGetLastError());
+}
+
 uint64_t round_state_bytes(uint32_t n, int32_t precision) {
     return ((uint64_t{1} << n) * amp_bytes(precision) + kAlign - 1) / kAlign * kAlign;
 }
 
 void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t threads, double* terms_out,
-                   RoundStats* stats, void* arena_in, const std::function<void(std::size_t, std::size_t)>& prepare) {
+                   RoundStats* stats, void* arena_in, const std::function<void(std::size_t, std::size_t)>& prepare,
+                   const BatchSchedule& schedule) {
     const std::size_t amp = amp_bytes(precision);
     const uint32_t width = window_width(precision);
     if (items.empty()) return;
@@ -157,7 +181,7 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
         // that only a sixteenth of the host work is exposed before the device
         // starts, and every later group's host work is shorter than the device
         // work queued before it.
-        BatchPassRunner runner(precision, threads);
+        BatchPassRunner runner(precision, threads, schedule);
         static constexpr std::size_t kEnds[] = {1, 2, 4, 8, 16};
         const std::size_t groups = count >= 64 ? 5 : 1;
         for (std::size_t g = 0; g < groups; ++g) {
@@ -289,7 +313,11 @@ extern "C" int mh_batch_execute(const mh_batch_request* req, mh_batch_result* re
                 });
                 plan_seconds += std::chrono::duration<double>(clock::now() - q0).count();
             };
-            execute_round(req->precision, items, threads, res->expectation_terms, &rs, nullptr, plan_group);
+            BatchSchedule schedule;
+            schedule.group_bytes = req->group_bytes;
+            if (req->kernel_streams != 0) schedule.kernel_streams = req->kernel_streams;
+            schedule.temporal = req->cached_states != 0;
+            execute_round(req->precision, items, threads, res->expectation_terms, &rs, nullptr, plan_group, schedule);
             device_seconds += std::chrono::duration<double>(clock::now() - r0).count();
             const double done = std::chrono::duration<double>(clock::now() - t0).count();
             if (res->completion_seconds)

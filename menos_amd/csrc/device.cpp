@@ -38,6 +38,8 @@ thread_local Streams* t_streams = nullptr;
 void create_streams(Streams& s) {
     MH_HIP(hipStreamCreateWithFlags(&s.main, hipStreamNonBlocking));
     MH_HIP(hipStreamCreateWithFlags(&s.second, hipStreamNonBlocking));
+    MH_HIP(hipStreamCreateWithFlags(&s.third, hipStreamNonBlocking));
+    MH_HIP(hipStreamCreateWithFlags(&s.fourth, hipStreamNonBlocking));
     MH_HIP(hipStreamCreateWithFlags(&s.copy, hipStreamNonBlocking));
 }
 
@@ -77,7 +79,7 @@ OwnStreams::OwnStreams() {
 
 OwnStreams::~OwnStreams() {
     t_streams = previous_;
-    for (hipStream_t s : {own_.copy, own_.second, own_.main}) {
+    for (hipStream_t s : {own_.copy, own_.fourth, own_.third, own_.second, own_.main}) {
         (void)hipStreamSynchronize(s);
         (void)hipStreamDestroy(s);
     }

@@ -590,21 +590,29 @@ ExpPlan plan_expectation(const std::vector<ExpectationItem>& items) {
     return P;
 }
 
-// Runs a plan with its workspace at `ws` (P.bytes bytes of device memory,
-// nothing in it need be zero: every partial and output is written before it is
-// read), then copies the terms to out[2 * term]. Uses the calling thread's main
-// stream and waits for it.
-template <typename V>
-void run_expectation(const ExpPlan& P, void* ws, double* out) {
-    if (P.slots.empty()) return;
+// Copies a plan's descriptors into its workspace (main stream, not waited for).
+void upload_expectation(const ExpPlan& P, void* ws) {
     char* w = static_cast<char*>(ws);
-    double* partials = reinterpret_cast<double*>(w + P.at_partials);
-    double* dout = reinterpret_cast<double*>(w + P.at_out);
     upload(w + P.at_slots, P.slots.data(), P.slots.size() * sizeof(ExpSlot));
     if (!P.tjobs.empty()) {
         upload(w + P.at_tjobs, P.tjobs.data(), P.tjobs.size() * sizeof(TileJob));
         upload(w + P.at_tterms, P.tterms.data(), P.tterms.size() * sizeof(TileTerm));
         upload(w + P.at_tbegin, P.tblock_begin.data(), P.tblock_begin.size() * sizeof(uint32_t));
+    }
+    if (!P.jobs.empty()) {
+        upload(w + P.at_jobs, P.jobs.data(), P.jobs.size() * sizeof(ExpJob));
+        upload(w + P.at_begin, P.block_begin.data(), P.block_begin.size() * sizeof(uint32_t));
+    }
+}
+
+// The reduction kernels of a plan whose descriptors are in the workspace
+// (main stream; no copies).
+template <typename V>
+void launch_expectation(const ExpPlan& P, void* ws) {
+    char* w = static_cast<char*>(ws);
+    double* partials = reinterpret_cast<double*>(w + P.at_partials);
+    double* dout = reinterpret_cast<double*>(w + P.at_out);
+    if (!P.tjobs.empty()) {
         hipLaunchKernelGGL(expectation_tile_kernel<V>, dim3(P.tile_blocks), dim3(kThreads), 0, streams().main,
                            reinterpret_cast<const TileJob*>(w + P.at_tjobs),
                            reinterpret_cast<const uint32_t*>(w + P.at_tbegin), static_cast<uint32_t>(P.tjobs.size()),
@@ -613,8 +621,6 @@ void run_expectation(const ExpPlan& P, void* ws, double* out) {
         MH_HIP(hipGetLastError());
     }
     if (!P.jobs.empty()) {
-        upload(w + P.at_jobs, P.jobs.data(), P.jobs.size() * sizeof(ExpJob));
-        upload(w + P.at_begin, P.block_begin.data(), P.block_begin.size() * sizeof(uint32_t));
         hipLaunchKernelGGL(expectation_batch_kernel<V>, dim3(P.total_blocks), dim3(kThreads), 0, streams().main,
                            reinterpret_cast<const ExpJob*>(w + P.at_jobs),
                            reinterpret_cast<const uint32_t*>(w + P.at_begin), static_cast<uint32_t>(P.jobs.size()),
@@ -625,6 +631,18 @@ void run_expectation(const ExpPlan& P, void* ws, double* out) {
                        streams().main, reinterpret_cast<const ExpSlot*>(w + P.at_slots),
                        static_cast<const double*>(partials), dout);
     MH_HIP(hipGetLastError());
+}
+
+// Runs a plan with its workspace at `ws` (P.bytes bytes of device memory,
+// nothing in it need be zero: every partial and output is written before it is
+// read), then copies the terms to out[2 * term]. Uses the calling thread's main
+// stream and waits for it.
+template <typename V>
+void run_expectation(const ExpPlan& P, void* ws, double* out) {
+    if (P.slots.empty()) return;
+    upload_expectation(P, ws);
+    launch_expectation<V>(P, ws);
+    const double* dout = reinterpret_cast<const double*>(static_cast<const char*>(ws) + P.at_out);
     std::vector<double> host(uint64_t(P.out_terms) * 2);
     download(host.data(), dout, host.size() * sizeof(double));
     for (const ExpSlot& sl : P.slots) {
@@ -641,6 +659,38 @@ void expectation_batch_typed(const std::vector<ExpectationItem>& items, double* 
     if (P.slots.empty()) return;
     DeviceBuffer ws(P.bytes);
     run_expectation<V>(P, ws.get(), out);
+}
+
+struct ExpectationProgram::Impl {
+    int32_t precision = MH_C128;
+    ExpPlan plan;
+};
+
+ExpectationProgram::ExpectationProgram(int32_t precision, const std::vector<ExpectationItem>& items)
+    : impl_(std::make_unique<Impl>()) {
+    amp_bytes(precision);
+    impl_->precision = precision;
+    impl_->plan = precision == MH_C128 ? plan_expectation<double2>(items) : plan_expectation<float2>(items);
+}
+
+ExpectationProgram::~ExpectationProgram() = default;
+
+uint64_t ExpectationProgram::workspace_bytes() const { return std::max<uint64_t>(impl_->plan.bytes, 256); }
+uint64_t ExpectationProgram::out_offset() const { return impl_->plan.at_out; }
+uint32_t ExpectationProgram::out_terms() const { return impl_->plan.out_terms; }
+
+void ExpectationProgram::upload(void* workspace) const {
+    if (impl_->plan.slots.empty()) return;
+    upload_expectation(impl_->plan, workspace);
+    synchronize();
+}
+
+void ExpectationProgram::launch(void* workspace) const {
+    if (impl_->plan.slots.empty()) return;
+    if (impl_->precision == MH_C128)
+        launch_expectation<double2>(impl_->plan, workspace);
+    else
+        launch_expectation<float2>(impl_->plan, workspace);
 }
 
 void expectation_batch(int32_t precision, const std::vector<ExpectationItem>& items, double* out) {

@@ -5,6 +5,7 @@
 
 #include <complex>
 #include <functional>
+#include <memory>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -40,12 +41,20 @@ int guarded(F&& f) noexcept {
     }
 }
 
-// A set of three streams of default priority: every kernel and copy the
-// library issues goes to the calling thread's set.
+// A set of streams of default priority: every kernel and copy the library
+// issues goes to the calling thread's set.
 struct Streams {
+    static constexpr uint32_t kKernelStreams = 4;
     hipStream_t main = nullptr;    // kernels
-    hipStream_t second = nullptr;  // a second kernel stream: batched groups alternate between the two
+    hipStream_t second = nullptr;  // more kernel streams: batched groups rotate over main, second, third, fourth
+    hipStream_t third = nullptr;
+    hipStream_t fourth = nullptr;
     hipStream_t copy = nullptr;    // table uploads that run ahead of the kernels
+
+    hipStream_t kernel(uint32_t i) const {
+        const hipStream_t k[kKernelStreams] = {main, second, third, fourth};
+        return k[i % kKernelStreams];
+    }
 };
 
 // The library's device context on device 0, created on first use, with the
@@ -164,6 +173,32 @@ struct ExpectationItem {
     uint32_t out_index = 0;
 };
 void expectation_batch(int32_t precision, const std::vector<ExpectationItem>& items, double* out);
+
+// The same reduction split for a caller that keeps it resident (a captured
+// graph): planned once on the host, its descriptors uploaded once into a
+// workspace of workspace_bytes(), then launched any number of times; each
+// launch leaves term slot t's (re, im) at workspace + out_offset() + 16 t.
+// States must stay where they were when it was planned.
+class ExpectationProgram {
+public:
+    ExpectationProgram(int32_t precision, const std::vector<ExpectationItem>& items);
+    ~ExpectationProgram();
+    ExpectationProgram(const ExpectationProgram&) = delete;
+    ExpectationProgram& operator=(const ExpectationProgram&) = delete;
+    uint64_t workspace_bytes() const;
+    uint64_t out_offset() const;
+    uint32_t out_terms() const;  // slots in the output region (the largest term index + 1)
+    // Copies the descriptors into the workspace on the calling thread's main
+    // stream and waits for the copy.
+    void upload(void* workspace) const;
+    // Launches the reduction kernels on the main stream (no copies, no
+    // allocation, no waits: capturable).
+    void launch(void* workspace) const;
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
 
 // ---- sampler (sampler.hip)
 void sample_sorted(const void* state, uint32_t n, int32_t precision, const double* sorted_u, uint64_t shots,

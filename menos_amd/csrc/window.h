@@ -45,6 +45,22 @@ struct BatchItem {
     uint32_t max_cost = 0;  // per-pass FP work cap; 0: kBatchPassCost
 };
 
+// How a batch's launches are laid out on the device.
+//   group_bytes = 0: pass p of every state of an add() call in one launch, the
+//   calls alternating between two kernel streams.
+//   group_bytes > 0: the states of an add() call are split, largest first,
+//   into cache groups of at most group_bytes of state (a state larger than
+//   that is a group of its own); each group runs all its passes back to back
+//   and the groups rotate over kernel_streams streams (1..4), so that a
+//   group's passes find its states in the 256 MB Infinity Cache.
+//   temporal: ordinary loads and stores of the states (they stay in the
+//   caches) instead of non-temporal ones.
+struct BatchSchedule {
+    uint64_t group_bytes = 0;
+    uint32_t kernel_streams = 2;
+    bool temporal = false;
+};
+
 struct BatchStats {
     uint32_t launches = 0;
     uint32_t max_passes = 0;
@@ -61,7 +77,7 @@ struct BatchStats {
 // what was launched.
 class BatchPassRunner {
 public:
-    BatchPassRunner(int32_t precision, uint32_t threads);
+    BatchPassRunner(int32_t precision, uint32_t threads, const BatchSchedule& schedule = {});
     ~BatchPassRunner();
     BatchPassRunner(const BatchPassRunner&) = delete;
     BatchPassRunner& operator=(const BatchPassRunner&) = delete;

@@ -43,6 +43,7 @@
 #include <thread>
 
 #include "mh_internal.h"
+#include "batch_program.h"
 #include "window.h"
 
 namespace mh {
@@ -117,6 +118,8 @@ bool uses_pairs(const uint64_t* bundles, std::size_t count, uint32_t rb) {
 static_assert(Stages::pair_reg(0, 3) < Stages::group(1) && Stages::pair_reg(3, 2) < Stages::dense2(4),
               "a register bit's stages fit its group");
 
+}  // namespace
+
 // The parameters of a stage that has them (blocks, not exchanges), in stage
 // order within a bundle. A sign stage applies a run of consecutive sign
 // diagonals at once: sign flips commute and compose exactly, so the run is
@@ -173,6 +176,8 @@ struct WBatchArgs {
     WLayout lay;
 };
 
+namespace {
+
 // The qubits outside the window, taken from the wave index (a bit deposit).
 __device__ __forceinline__ uint64_t outer_base(uint32_t wave, const WLayout& lay) {
     uint64_t base = 0;
@@ -207,6 +212,24 @@ __device__ __forceinline__ void store_amp(float2* p, float2 v) {
     x[0] = v.x;
     x[1] = v.y;
     __builtin_nontemporal_store(x, reinterpret_cast<VF2*>(p));
+}
+
+// Ordinary (cached) accesses, for states small enough to stay in the caches
+// from one pass to the next.
+template <bool kTemporal, typename V>
+__device__ __forceinline__ V load_state(const V* p) {
+    if constexpr (kTemporal)
+        return *p;
+    else
+        return load_amp(p);
+}
+
+template <bool kTemporal, typename V>
+__device__ __forceinline__ void store_state(V* p, V v) {
+    if constexpr (kTemporal)
+        *p = v;
+    else
+        store_amp(p, v);
 }
 
 __device__ __forceinline__ uint64_t lane_offset(uint32_t lane, const uint8_t* q) {
@@ -546,7 +569,7 @@ __device__ __forceinline__ void reg_swap_stage(uint64_t c, V* v) {
 }
 
 // One wave's window of one pass.
-template <typename R, bool kPairs>
+template <typename R, bool kPairs, bool kTemporal = false>
 __device__ __forceinline__ void window_body(typename Shape<R>::V* state, const uint64_t* bundles, const WOp* ops,
                                             const SignTerm* terms, const typename Shape<R>::V* mats,
                                             const WLayout& a, uint32_t wave, uint32_t lane) {
@@ -568,7 +591,7 @@ __device__ __forceinline__ void window_body(typename Shape<R>::V* state, const u
 #pragma unroll
             for (uint32_t j = 0; j < RB; ++j)
                 if ((r >> j) & 1u) h |= rq[j];
-            v[r] = load_amp(src + h);
+            v[r] = load_state<kTemporal>(src + h);
         }
     }
 
@@ -621,7 +644,7 @@ __device__ __forceinline__ void window_body(typename Shape<R>::V* state, const u
 #pragma unroll
             for (uint32_t j = 0; j < RB; ++j)
                 if ((r >> j) & 1u) h |= rq[j];
-            store_amp(dst + h, v[r]);
+            store_state<kTemporal>(dst + h, v[r]);
         }
     }
 }
@@ -640,7 +663,7 @@ __global__ void __launch_bounds__(kWavesPerGroup * 64, 4) window_kernel(WPassArg
 // One pass of many states in one launch: workgroup b belongs to the state c
 // with group_begin[c] <= b < group_begin[c + 1] (each state's windows in
 // groups of four waves); a wave past its state's last window returns.
-template <typename R, bool kPairs>
+template <typename R, bool kPairs, bool kTemporal>
 __global__ void __launch_bounds__(kWavesPerGroup * 64, 4)
     window_batch_kernel(const WBatchArgs* __restrict__ args, const uint32_t* __restrict__ group_begin, uint32_t count,
                         typename Shape<R>::V* state, const uint64_t* __restrict__ bundles,
@@ -660,7 +683,7 @@ __global__ void __launch_bounds__(kWavesPerGroup * 64, 4)
     const uint32_t wave =
         __builtin_amdgcn_readfirstlane((b - group_begin[lo]) * kWavesPerGroup + (threadIdx.x >> 6));
     if (wave >= a.lay.num_waves) return;
-    window_body<R, kPairs>(state + a.state_off, bundles + a.bundle_off, ops + a.op_off, terms + a.term_off,
+    window_body<R, kPairs, kTemporal>(state + a.state_off, bundles + a.bundle_off, ops + a.op_off, terms + a.term_off,
                            mats + a.mat_off, a.lay, wave, threadIdx.x & 63u);
 }
 
@@ -1203,40 +1226,86 @@ struct BatchKeep {
     hipEvent_t copied = nullptr;
 };
 
-template <typename T>
-void keep_upload(BatchKeep& keep, DeviceBuffer& dst, std::vector<T>&& src, hipStream_t copy_stream) {
-    auto held = std::make_shared<std::vector<T>>(std::move(src));
-    MH_HIP(hipMemcpyAsync(dst.get(), held->data(), held->size() * sizeof(T), hipMemcpyHostToDevice, copy_stream));
-    keep.host.push_back(held);
+
+// What compile_batch_program keeps per state: where each pass's program
+// starts, or which block a generic pass applies.
+struct BatchPassRec {
+    WLayout lay;
+    uint32_t bundle_begin = 0, op_begin = 0;
+    bool generic = false;
+    uint32_t generic_block = 0;
+};
+
+}  // namespace
+
+struct BatchProgramData {
+    int32_t precision = MH_C128;
+    std::vector<unsigned char> bundles, ops, terms, mats, args, groups;
+    std::vector<std::vector<BatchPassRec>> passes;   // per state
+    std::vector<std::vector<std::size_t>> lgroups;   // launch groups of states
+    std::vector<BatchLaunch> launches;
+    bool has_generic = false;
+    uint32_t max_passes = 0;
+};
+
+BatchProgram::BatchProgram() : d_(new BatchProgramData()) {}
+BatchProgram::~BatchProgram() { delete d_; }
+BatchProgram::BatchProgram(BatchProgram&& o) noexcept : d_(o.d_) { o.d_ = nullptr; }
+BatchProgram& BatchProgram::operator=(BatchProgram&& o) noexcept {
+    if (this != &o) {
+        delete d_;
+        d_ = o.d_;
+        o.d_ = nullptr;
+    }
+    return *this;
+}
+int32_t BatchProgram::precision() const { return d_->precision; }
+const std::vector<unsigned char>& BatchProgram::bundles() const { return d_->bundles; }
+const std::vector<unsigned char>& BatchProgram::ops() const { return d_->ops; }
+const std::vector<unsigned char>& BatchProgram::terms() const { return d_->terms; }
+const std::vector<unsigned char>& BatchProgram::mats() const { return d_->mats; }
+const std::vector<unsigned char>& BatchProgram::args() const { return d_->args; }
+const std::vector<unsigned char>& BatchProgram::groups() const { return d_->groups; }
+bool BatchProgram::has_generic() const { return d_->has_generic; }
+uint32_t BatchProgram::launch_count() const { return static_cast<uint32_t>(d_->launches.size()); }
+BatchProgramData& BatchProgram::data() { return *d_; }
+const BatchProgramData& BatchProgram::data() const { return *d_; }
+
+bool BatchProgram::same_shape(const BatchProgram& o) const {
+    const BatchProgramData& a = *d_;
+    const BatchProgramData& b = *o.d_;
+    return a.precision == b.precision && a.bundles == b.bundles && a.ops == b.ops && a.terms == b.terms &&
+           a.args == b.args && a.groups == b.groups && a.mats.size() == b.mats.size() &&
+           a.has_generic == b.has_generic && a.launches.size() == b.launches.size();
 }
 
-// Many states, each with its own block list: pass p of every state runs in
-// one launch of window_batch_kernel (blocks the window kernel cannot apply run
-// on the generic kernel, state by state, after that launch). Returns without
-// waiting: the tables go up on the copy stream, the launches wait for them on
-// `stream` (the main stream when a pass is generic), and `keep` holds what
-// they read.
+std::size_t wop_bytes() { return sizeof(WOp); }
+std::size_t sign_term_bytes() { return sizeof(SignTerm); }
+std::size_t wbatch_args_bytes() { return sizeof(WBatchArgs); }
+
+namespace {
+
+template <typename T>
+void append_bytes(std::vector<unsigned char>& dst, const std::vector<T>& src) {
+    const std::size_t at = dst.size();
+    dst.resize(at + src.size() * sizeof(T));
+    if (!src.empty()) std::memcpy(dst.data() + at, src.data(), src.size() * sizeof(T));
+}
+
 template <typename R>
-void launch_window_passes_batch(const BatchItem* items, std::size_t count, uint32_t threads, BatchStats* stats,
-                                BatchKeep& keep, hipStream_t stream) {
+void compile_batch_typed(const BatchItem* items, std::size_t count, uint32_t threads, const BatchSchedule& sched,
+                         BatchProgramData& prog) {
     using V = typename Shape<R>::V;
     constexpr uint32_t W = width_of<R>();
-    context();
-    struct PassRec {
-        WPassArgs args;
-        uint32_t bundle_begin = 0, op_begin = 0;
-        bool generic = false;
-        uint32_t generic_block = 0;
-    };
     struct Compiled {
         std::vector<uint64_t> bundles;
         std::vector<WOp> ops;
         std::vector<SignTerm> terms;
         std::vector<V> mats;
-        std::vector<PassRec> passes;
     };
-    const auto c0 = std::chrono::steady_clock::now();
+    prog.precision = sizeof(R) == 8 ? MH_C128 : MH_C64;
     std::vector<Compiled> comp(count);
+    prog.passes.assign(count, {});
     std::atomic<std::size_t> next{0};
     std::exception_ptr error;
     std::mutex error_mutex;
@@ -1246,22 +1315,21 @@ void launch_window_passes_batch(const BatchItem* items, std::size_t count, uint3
                 const BatchItem& it = items[i];
                 Compiled& c = comp[i];
                 const std::vector<WindowPass> passes =
-                    plan_window_passes(it.n, sizeof(R) == 8 ? MH_C128 : MH_C64, *it.blocks, W,
-                                       it.max_cost ? it.max_cost : kBatchPassCost);
+                    plan_window_passes(it.n, prog.precision, *it.blocks, W, it.max_cost ? it.max_cost : kBatchPassCost);
                 PassCompiler<R> compiler(c.bundles, c.ops, c.terms, c.mats);
                 for (std::size_t p = 0; p < passes.size(); ++p) {
-                    PassRec rec;
+                    BatchPassRec rec;
                     rec.bundle_begin = static_cast<uint32_t>(c.bundles.size());
                     rec.op_begin = static_cast<uint32_t>(c.ops.size());
                     if (passes[p].generic) {
                         rec.generic = true;
                         rec.generic_block = passes[p].blocks[0];
                     } else {
-                        rec.args = compiler.compile(it.n, *it.blocks, passes[p], p + 1 == passes.size() && sizeof(R) == 8);
-                        rec.args.lay.num_bundles = static_cast<uint32_t>(c.bundles.size()) - rec.bundle_begin;
-                        rec.args.lay.num_waves = 1u << (it.n - W);
+                        rec.lay = compiler.compile(it.n, *it.blocks, passes[p], p + 1 == passes.size() && sizeof(R) == 8).lay;
+                        rec.lay.num_bundles = static_cast<uint32_t>(c.bundles.size()) - rec.bundle_begin;
+                        rec.lay.num_waves = 1u << (it.n - W);
                     }
-                    c.passes.push_back(rec);
+                    prog.passes[i].push_back(rec);
                 }
             }
         } catch (...) {
@@ -1278,9 +1346,10 @@ void launch_window_passes_batch(const BatchItem* items, std::size_t count, uint3
     }
     if (error) std::rethrow_exception(error);
 
-    // One device copy of every state's tables.
+    // Every state's tables, packed one after another.
     std::vector<std::size_t> bundle_base(count), op_base(count), term_base(count), mat_base(count);
-    std::size_t nb = 0, no = 0, nt = 0, nm = 0, max_passes = 0;
+    std::size_t nb = 0, no = 0, nt = 0, nm = 0;
+    prog.max_passes = 0;
     for (std::size_t i = 0; i < count; ++i) {
         bundle_base[i] = nb;
         op_base[i] = no;
@@ -1290,117 +1359,215 @@ void launch_window_passes_batch(const BatchItem* items, std::size_t count, uint3
         no += comp[i].ops.size();
         nt += comp[i].terms.size();
         nm += comp[i].mats.size();
-        max_passes = std::max(max_passes, comp[i].passes.size());
+        prog.max_passes = std::max(prog.max_passes, static_cast<uint32_t>(prog.passes[i].size()));
     }
-    std::vector<uint64_t> bundles(std::max<std::size_t>(nb, 1));
-    std::vector<WOp> ops(std::max<std::size_t>(no, 1));
-    std::vector<SignTerm> terms(std::max<std::size_t>(nt, 1));
-    std::vector<V> mats(std::max<std::size_t>(nm, 1));
+    prog.bundles.clear();
+    prog.ops.clear();
+    prog.terms.clear();
+    prog.mats.clear();
+    prog.bundles.reserve(std::max<std::size_t>(nb, 1) * sizeof(uint64_t));
+    prog.ops.reserve(std::max<std::size_t>(no, 1) * sizeof(WOp));
+    prog.terms.reserve(std::max<std::size_t>(nt, 1) * sizeof(SignTerm));
+    prog.mats.reserve(std::max<std::size_t>(nm, 1) * sizeof(V));
     for (std::size_t i = 0; i < count; ++i) {
-        std::copy(comp[i].bundles.begin(), comp[i].bundles.end(), bundles.begin() + bundle_base[i]);
-        std::copy(comp[i].ops.begin(), comp[i].ops.end(), ops.begin() + op_base[i]);
-        std::copy(comp[i].terms.begin(), comp[i].terms.end(), terms.begin() + term_base[i]);
-        std::copy(comp[i].mats.begin(), comp[i].mats.end(), mats.begin() + mat_base[i]);
+        append_bytes(prog.bundles, comp[i].bundles);
+        append_bytes(prog.ops, comp[i].ops);
+        append_bytes(prog.terms, comp[i].terms);
+        append_bytes(prog.mats, comp[i].mats);
     }
-    DeviceBuffer dbundles(bundles.size() * sizeof(uint64_t)), dops(ops.size() * sizeof(WOp));
-    DeviceBuffer dterms(terms.size() * sizeof(SignTerm)), dmats(mats.size() * sizeof(V));
-    const uint64_t* const dbundles_p = dbundles.as<uint64_t>();
-    const WOp* const dops_p = dops.as<WOp>();
-    const SignTerm* const dterms_p = dterms.as<SignTerm>();
-    const V* const dmats_p = dmats.as<V>();
-    keep_upload(keep, dbundles, std::move(bundles), streams().copy);
-    keep_upload(keep, dops, std::move(ops), streams().copy);
-    keep_upload(keep, dterms, std::move(terms), streams().copy);
-    keep_upload(keep, dmats, std::move(mats), streams().copy);
+    // Tables are never empty (a zero-length device buffer has no address).
+    if (prog.bundles.empty()) prog.bundles.resize(sizeof(uint64_t));
+    if (prog.ops.empty()) prog.ops.resize(sizeof(WOp));
+    if (prog.terms.empty()) prog.terms.resize(sizeof(SignTerm));
+    if (prog.mats.empty()) prog.mats.resize(sizeof(V));
 
-    // Launch lists: pass p of every state that has one. State offsets count
-    // amplitudes from the lowest state address.
+    // Launch groups: every state in one group (pass p of all of them per
+    // launch), or cache groups of at most sched.group_bytes of state, largest
+    // states first, each running its passes back to back.
+    prog.lgroups.clear();
+    if (sched.group_bytes == 0) {
+        prog.lgroups.emplace_back(count);
+        for (std::size_t i = 0; i < count; ++i) prog.lgroups[0][i] = i;
+    } else {
+        std::vector<std::size_t> order(count);
+        for (std::size_t i = 0; i < count; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](std::size_t x, std::size_t y) { return items[x].n > items[y].n; });
+        uint64_t held = 0;
+        for (std::size_t i : order) {
+            const uint64_t bytes = (uint64_t{1} << items[i].n) * sizeof(V);
+            if (prog.lgroups.empty() || held + bytes > sched.group_bytes) {
+                prog.lgroups.emplace_back();
+                held = 0;
+            }
+            prog.lgroups.back().push_back(i);
+            held += bytes;
+        }
+    }
+    // Launch lists: pass p of every state of a group that has one. State
+    // offsets count amplitudes from the lowest state address.
     uintptr_t base_addr = UINTPTR_MAX;
     for (std::size_t i = 0; i < count; ++i) base_addr = std::min(base_addr, reinterpret_cast<uintptr_t>(items[i].state));
-    V* const state_base = reinterpret_cast<V*>(base_addr);
     std::vector<WBatchArgs> all_args;
     std::vector<uint32_t> all_groups;
-    std::vector<std::size_t> args_at(max_passes), groups_at(max_passes), list_len(max_passes);
-    std::vector<uint32_t> total_groups(max_passes);
-    std::vector<char> pairs_in(max_passes, 0);
-    for (std::size_t p = 0; p < max_passes; ++p) {
-        args_at[p] = all_args.size();
-        groups_at[p] = all_groups.size();
-        uint32_t g = 0;
-        for (std::size_t i = 0; i < count; ++i) {
-            if (p >= comp[i].passes.size() || comp[i].passes[p].generic) continue;
-            const PassRec& rec = comp[i].passes[p];
-            const uintptr_t addr = reinterpret_cast<uintptr_t>(items[i].state);
-            if ((addr - base_addr) % sizeof(V) != 0) fail(MH_ERR_INVALID, "a batched state is not aligned to its amplitudes");
-            WBatchArgs a;
-            std::memset(&a, 0, sizeof(a));
-            a.state_off = (addr - base_addr) / sizeof(V);
-            a.bundle_off = static_cast<uint32_t>(bundle_base[i] + rec.bundle_begin);
-            a.op_off = static_cast<uint32_t>(op_base[i] + rec.op_begin);
-            a.term_off = static_cast<uint32_t>(term_base[i]);
-            a.mat_off = static_cast<uint32_t>(mat_base[i]);
-            a.lay = rec.args.lay;
-            if (uses_pairs(comp[i].bundles.data() + rec.bundle_begin, rec.args.lay.num_bundles, Shape<R>::kRegBits))
-                pairs_in[p] = 1;
-            all_args.push_back(a);
-            all_groups.push_back(g);
-            g += (a.lay.num_waves + kWavesPerGroup - 1) / kWavesPerGroup;
+    prog.launches.clear();
+    prog.has_generic = false;
+    for (uint32_t lg = 0; lg < prog.lgroups.size(); ++lg) {
+        std::size_t group_passes = 0;
+        for (std::size_t i : prog.lgroups[lg]) group_passes = std::max(group_passes, prog.passes[i].size());
+        for (std::size_t p = 0; p < group_passes; ++p) {
+            BatchLaunch l;
+            l.args_at = all_args.size();
+            l.groups_at = all_groups.size();
+            l.group = lg;
+            l.pass = static_cast<uint32_t>(p);
+            uint32_t g = 0;
+            for (std::size_t i : prog.lgroups[lg]) {
+                if (p >= prog.passes[i].size()) continue;
+                const BatchPassRec& rec = prog.passes[i][p];
+                if (rec.generic) {
+                    prog.has_generic = true;
+                    continue;
+                }
+                const uintptr_t addr = reinterpret_cast<uintptr_t>(items[i].state);
+                if ((addr - base_addr) % sizeof(V) != 0)
+                    fail(MH_ERR_INVALID, "a batched state is not aligned to its amplitudes");
+                WBatchArgs a;
+                std::memset(&a, 0, sizeof(a));
+                a.state_off = (addr - base_addr) / sizeof(V);
+                a.bundle_off = static_cast<uint32_t>(bundle_base[i] + rec.bundle_begin);
+                a.op_off = static_cast<uint32_t>(op_base[i] + rec.op_begin);
+                a.term_off = static_cast<uint32_t>(term_base[i]);
+                a.mat_off = static_cast<uint32_t>(mat_base[i]);
+                a.lay = rec.lay;
+                if (uses_pairs(comp[i].bundles.data() + rec.bundle_begin, rec.lay.num_bundles, Shape<R>::kRegBits))
+                    l.pairs = true;
+                all_args.push_back(a);
+                all_groups.push_back(g);
+                g += (a.lay.num_waves + kWavesPerGroup - 1) / kWavesPerGroup;
+            }
+            l.len = all_args.size() - l.args_at;
+            l.total_groups = g;
+            prog.launches.push_back(l);
         }
-        list_len[p] = all_args.size() - args_at[p];
-        total_groups[p] = g;
     }
-    DeviceBuffer dargs(std::max<std::size_t>(all_args.size(), 1) * sizeof(WBatchArgs));
-    DeviceBuffer dgroups(std::max<std::size_t>(all_groups.size(), 1) * sizeof(uint32_t));
-    const WBatchArgs* const dargs_p = dargs.as<WBatchArgs>();
-    const uint32_t* const dgroups_p = dgroups.as<uint32_t>();
-    keep_upload(keep, dargs, std::move(all_args), streams().copy);
-    keep_upload(keep, dgroups, std::move(all_groups), streams().copy);
-    MH_HIP(hipEventCreateWithFlags(&keep.copied, hipEventDisableTiming));
-    MH_HIP(hipEventRecord(keep.copied, streams().copy));
-    // A generic pass runs on the main stream (launch_generic), so a group that
-    // has one stays there in full.
-    bool has_generic = false;
-    for (std::size_t i = 0; i < count && !has_generic; ++i)
-        for (const PassRec& rec : comp[i].passes) has_generic = has_generic || rec.generic;
-    if (has_generic) stream = streams().main;
-    MH_HIP(hipStreamWaitEvent(stream, keep.copied, 0));
-    keep.device.push_back(std::move(dbundles));
-    keep.device.push_back(std::move(dops));
-    keep.device.push_back(std::move(dterms));
-    keep.device.push_back(std::move(dmats));
-    keep.device.push_back(std::move(dargs));
-    keep.device.push_back(std::move(dgroups));
+    if (all_args.empty()) all_args.resize(1);
+    if (all_groups.empty()) all_groups.resize(1);
+    prog.args.clear();
+    prog.groups.clear();
+    append_bytes(prog.args, all_args);
+    append_bytes(prog.groups, all_groups);
+}
 
-    if (stats) stats->compile_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
+template <typename R>
+uint32_t issue_batch_typed(const BatchProgramData& prog, const BatchItem* items, const BatchDeviceTables& dev,
+                           const BatchSchedule& sched, uint32_t first_stream) {
+    using V = typename Shape<R>::V;
+    uintptr_t base_addr = UINTPTR_MAX;
+    for (std::size_t i = 0; i < prog.passes.size(); ++i)
+        base_addr = std::min(base_addr, reinterpret_cast<uintptr_t>(items[i].state));
+    V* const state_base = reinterpret_cast<V*>(base_addr);
+    const V* const mats = static_cast<const V*>(dev.mats);
+    const uint32_t nstreams =
+        prog.has_generic ? 1u : std::max(1u, std::min(sched.kernel_streams, Streams::kKernelStreams));
     uint32_t launches = 0;
-    for (std::size_t p = 0; p < max_passes; ++p) {
-        if (list_len[p] > 0) {
-            if (pairs_in[p])
-                hipLaunchKernelGGL((window_batch_kernel<R, true>), dim3(total_groups[p]), dim3(kWavesPerGroup * 64), 0,
-                                   stream, dargs_p + args_at[p], dgroups_p + groups_at[p],
-                                   static_cast<uint32_t>(list_len[p]), state_base, dbundles_p, dops_p, dterms_p, dmats_p);
+    for (const BatchLaunch& l : prog.launches) {
+        const hipStream_t stream = prog.has_generic ? streams().main : streams().kernel((first_stream + l.group) % nstreams);
+        if (l.len > 0) {
+            const dim3 grid(l.total_groups), block(kWavesPerGroup * 64);
+            const WBatchArgs* args_p = dev.args + l.args_at;
+            const uint32_t* groups_p = dev.groups + l.groups_at;
+            const uint32_t len = static_cast<uint32_t>(l.len);
+            if (l.pairs && sched.temporal)
+                hipLaunchKernelGGL((window_batch_kernel<R, true, true>), grid, block, 0, stream, args_p, groups_p, len,
+                                   state_base, dev.bundles, dev.ops, dev.terms, mats);
+            else if (l.pairs)
+                hipLaunchKernelGGL((window_batch_kernel<R, true, false>), grid, block, 0, stream, args_p, groups_p, len,
+                                   state_base, dev.bundles, dev.ops, dev.terms, mats);
+            else if (sched.temporal)
+                hipLaunchKernelGGL((window_batch_kernel<R, false, true>), grid, block, 0, stream, args_p, groups_p, len,
+                                   state_base, dev.bundles, dev.ops, dev.terms, mats);
             else
-                hipLaunchKernelGGL((window_batch_kernel<R, false>), dim3(total_groups[p]), dim3(kWavesPerGroup * 64), 0,
-                                   stream, dargs_p + args_at[p], dgroups_p + groups_at[p],
-                                   static_cast<uint32_t>(list_len[p]), state_base, dbundles_p, dops_p, dterms_p, dmats_p);
+                hipLaunchKernelGGL((window_batch_kernel<R, false, false>), grid, block, 0, stream, args_p, groups_p, len,
+                                   state_base, dev.bundles, dev.ops, dev.terms, mats);
             MH_HIP(hipGetLastError());
             ++launches;
         }
-        for (std::size_t i = 0; i < count; ++i) {
-            if (p >= comp[i].passes.size() || !comp[i].passes[p].generic) continue;
-            const Block& blk = (*items[i].blocks)[comp[i].passes[p].generic_block];
-            launch_generic(items[i].state, items[i].n, sizeof(R) == 8 ? MH_C128 : MH_C64, blk.matrix.data(),
-                           blk.operands.data(), static_cast<uint32_t>(blk.operands.size()));
+        for (std::size_t i : prog.lgroups[l.group]) {
+            if (l.pass >= prog.passes[i].size() || !prog.passes[i][l.pass].generic) continue;
+            const Block& blk = (*items[i].blocks)[prog.passes[i][l.pass].generic_block];
+            launch_generic(items[i].state, items[i].n, prog.precision, blk.matrix.data(), blk.operands.data(),
+                           static_cast<uint32_t>(blk.operands.size()));
             ++launches;
         }
     }
+    return launches;
+}
+
+// A device copy of a table kept with the group that reads it.
+void keep_table(BatchKeep& keep, const std::vector<unsigned char>& src, hipStream_t copy_stream, const void*& out) {
+    DeviceBuffer d(src.size());
+    auto held = std::make_shared<std::vector<unsigned char>>(src);
+    MH_HIP(hipMemcpyAsync(d.get(), held->data(), held->size(), hipMemcpyHostToDevice, copy_stream));
+    keep.host.push_back(held);
+    out = d.get();
+    keep.device.push_back(std::move(d));
+}
+
+// Many states, each with its own block list, compiled into one program whose
+// tables go up on the copy stream; its launches then wait for them on the
+// kernel streams (see issue_batch_program). Returns without waiting; `keep`
+// holds what the launches read.
+void launch_window_passes_batch(int32_t precision, const BatchItem* items, std::size_t count, uint32_t threads,
+                                BatchStats* stats, BatchKeep& keep, const BatchSchedule& sched, uint32_t first_stream) {
+    context();
+    const auto c0 = std::chrono::steady_clock::now();
+    const BatchProgram prog = compile_batch_program(precision, items, count, threads, sched);
+    BatchDeviceTables dev;
+    const void* p = nullptr;
+    keep_table(keep, prog.bundles(), streams().copy, p);
+    dev.bundles = static_cast<const uint64_t*>(p);
+    keep_table(keep, prog.ops(), streams().copy, p);
+    dev.ops = static_cast<const WOp*>(p);
+    keep_table(keep, prog.terms(), streams().copy, p);
+    dev.terms = static_cast<const SignTerm*>(p);
+    keep_table(keep, prog.mats(), streams().copy, dev.mats);
+    keep_table(keep, prog.args(), streams().copy, p);
+    dev.args = static_cast<const WBatchArgs*>(p);
+    keep_table(keep, prog.groups(), streams().copy, p);
+    dev.groups = static_cast<const uint32_t*>(p);
+    MH_HIP(hipEventCreateWithFlags(&keep.copied, hipEventDisableTiming));
+    MH_HIP(hipEventRecord(keep.copied, streams().copy));
+    if (prog.has_generic()) {
+        MH_HIP(hipStreamWaitEvent(streams().main, keep.copied, 0));
+    } else {
+        const uint32_t nstreams = std::max(1u, std::min(sched.kernel_streams, Streams::kKernelStreams));
+        for (uint32_t k = 0; k < nstreams; ++k) MH_HIP(hipStreamWaitEvent(streams().kernel(k), keep.copied, 0));
+    }
+    if (stats) stats->compile_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
+    const uint32_t launches = issue_batch_program(prog, items, dev, sched, first_stream);
     if (stats) {
         stats->launches += launches;
-        stats->max_passes = std::max(stats->max_passes, static_cast<uint32_t>(max_passes));
+        stats->max_passes = std::max(stats->max_passes, prog.data().max_passes);
     }
 }
 
 }  // namespace
+
+BatchProgram compile_batch_program(int32_t precision, const BatchItem* items, std::size_t count, uint32_t threads,
+                                   const BatchSchedule& sched) {
+    BatchProgram prog;
+    if (precision == MH_C128)
+        compile_batch_typed<double>(items, count, threads, sched, prog.data());
+    else
+        compile_batch_typed<float>(items, count, threads, sched, prog.data());
+    return prog;
+}
+
+uint32_t issue_batch_program(const BatchProgram& prog, const BatchItem* items, const BatchDeviceTables& dev,
+                             const BatchSchedule& sched, uint32_t first_stream) {
+    if (prog.precision() == MH_C128) return issue_batch_typed<double>(prog.data(), items, dev, sched, first_stream);
+    return issue_batch_typed<float>(prog.data(), items, dev, sched, first_stream);
+}
 
 uint32_t window_width(int32_t precision) noexcept {
     return precision == MH_C128 ? width_of<double>() : width_of<float>();
@@ -1463,27 +1630,29 @@ std::vector<WindowPass> plan_window_passes(uint32_t n, int32_t precision, const 
 struct BatchPassRunner::Impl {
     int32_t precision = MH_C128;
     uint32_t threads = 1;
+    BatchSchedule schedule;
     uint32_t added = 0;
     hipEvent_t start = nullptr;  // the work queued before the runner, for the second stream to wait on
     std::vector<BatchKeep> keeps;
 };
 
-BatchPassRunner::BatchPassRunner(int32_t precision, uint32_t threads) : impl_(std::make_unique<Impl>()) {
+BatchPassRunner::BatchPassRunner(int32_t precision, uint32_t threads, const BatchSchedule& schedule)
+    : impl_(std::make_unique<Impl>()) {
     amp_bytes(precision);
     impl_->precision = precision;
     impl_->threads = threads;
+    impl_->schedule = schedule;
     context();
     MH_HIP(hipEventCreateWithFlags(&impl_->start, hipEventDisableTiming));
     MH_HIP(hipEventRecord(impl_->start, streams().main));
-    MH_HIP(hipStreamWaitEvent(streams().second, impl_->start, 0));
+    for (uint32_t k = 1; k < Streams::kKernelStreams; ++k) MH_HIP(hipStreamWaitEvent(streams().kernel(k), impl_->start, 0));
 }
 
 BatchPassRunner::~BatchPassRunner() {
     // Whatever was launched or copied reads the kept tables: wait before they go.
     if (!impl_->keeps.empty()) {
         (void)hipStreamSynchronize(streams().copy);
-        (void)hipStreamSynchronize(streams().second);
-        (void)hipStreamSynchronize(streams().main);
+        for (uint32_t k = 0; k < Streams::kKernelStreams; ++k) (void)hipStreamSynchronize(streams().kernel(k));
     }
     for (BatchKeep& k : impl_->keeps)
         if (k.copied) (void)hipEventDestroy(k.copied);
@@ -1496,19 +1665,17 @@ void BatchPassRunner::add(const BatchItem* items, std::size_t count, BatchStats*
     for (std::size_t i = 0; i < count; ++i)
         if (items[i].n < width) fail(MH_ERR_INVALID, "a batched state has fewer qubits than one window");
     impl_->keeps.emplace_back();
-    // Groups alternate between the two kernel streams, so the short launches at
-    // the end of one group overlap the next group's.
+    // Calls alternate between the kernel streams, so the short launches at
+    // the end of one call overlap the next call's.
     context();
-    const hipStream_t stream = (impl_->added++ % 2 == 0) ? streams().main : streams().second;
-    if (impl_->precision == MH_C128)
-        launch_window_passes_batch<double>(items, count, impl_->threads, stats, impl_->keeps.back(), stream);
-    else
-        launch_window_passes_batch<float>(items, count, impl_->threads, stats, impl_->keeps.back(), stream);
+    const uint32_t first_stream = impl_->added++;
+    launch_window_passes_batch(impl_->precision, items, count, impl_->threads, stats, impl_->keeps.back(),
+                               impl_->schedule, first_stream);
 }
 
 void BatchPassRunner::finish() {
     MH_HIP(hipStreamSynchronize(streams().copy));
-    MH_HIP(hipStreamSynchronize(streams().second));
+    for (uint32_t k = 1; k < Streams::kKernelStreams; ++k) MH_HIP(hipStreamSynchronize(streams().kernel(k)));
     synchronize();
     for (BatchKeep& k : impl_->keeps)
         if (k.copied) (void)hipEventDestroy(k.copied);

@@ -32,7 +32,8 @@ EXPORTS = (
     "mh_gate_matrix", "mh_calibrate", "mh_batch_execute", "mh_state_max_abs_diff", "mh_server_create",
     "mh_server_submit", "mh_server_wait", "mh_server_replay", "mh_server_stats_get", "mh_server_destroy",
     "mh_derive_seeds", "mh_set_calibration", "mh_get_calibration", "mh_expectation_noalloc",
-    "mh_plan_passes", "mh_gather",
+    "mh_plan_passes", "mh_gather", "mh_batch_graph_create", "mh_batch_graph_update", "mh_batch_graph_run",
+    "mh_batch_graph_destroy",
 )
 JOB_ESTIMATOR, JOB_SAMPLER = 0, 1
 
@@ -386,7 +387,9 @@ class CBatchRequest(ctypes.Structure):
                 ("precision", ctypes.c_int32), ("fusion", ctypes.c_int32), ("fusion_width", ctypes.c_uint32),
                 ("global_phases", _f64p), ("term_offsets", _u32p), ("x_masks", _u64p), ("z_masks", _u64p),
                 ("round_bytes", ctypes.c_uint64), ("host_threads", ctypes.c_uint32),
-                ("want_statevectors", ctypes.c_int32), ("max_pass_cost", ctypes.c_uint32)]
+                ("want_statevectors", ctypes.c_int32), ("max_pass_cost", ctypes.c_uint32),
+                ("group_bytes", ctypes.c_uint64), ("kernel_streams", ctypes.c_uint32),
+                ("cached_states", ctypes.c_int32)]
 
 
 class CBatchResult(ctypes.Structure):
@@ -397,7 +400,8 @@ class CBatchResult(ctypes.Structure):
 
 
 def batch_execute(circuits, terms, *, precision=C128, fusion=FUSE_CONTAINMENT, fusion_width=0, round_bytes=0,
-                  host_threads=0, statevectors=False, max_pass_cost=0):
+                  host_threads=0, statevectors=False, max_pass_cost=0, group_bytes=0, kernel_streams=0,
+                  cached_states=False):
     """Runs every circuit and reduces its Pauli terms (a list of (x, z) masks per
     circuit) through mh_batch_execute. Returns a Result with `expectation` (one
     complex array per circuit), `completion_seconds`, optionally `statevectors`
@@ -416,7 +420,7 @@ def batch_execute(circuits, terms, *, precision=C128, fusion=FUSE_CONTAINMENT, f
     phases = np.array([c.global_phase for c in circuits] or [0.0], np.float64)
     req = CBatchRequest(len(circuits), arr, precision, fusion, fusion_width, _ptr(phases, _f64p), _ptr(offs, _u32p),
                         _ptr(xs, _u64p), _ptr(zs, _u64p), round_bytes, host_threads, 1 if statevectors else 0,
-                        max_pass_cost)
+                        max_pass_cost, group_bytes, kernel_streams, 1 if cached_states else 0)
     res = CBatchResult(_ptr(out, _f64p), sv.ctypes.data_as(ctypes.c_void_p), _ptr(done, _f64p))
     check(lib().mh_batch_execute(ctypes.byref(req), ctypes.byref(res)))
     e = out[:2 * int(offs[-1])].view(np.complex128)
@@ -430,6 +434,61 @@ def batch_execute(circuits, terms, *, precision=C128, fusion=FUSE_CONTAINMENT, f
             r.statevectors.append(sv[at:at + s].copy())
             at += s
     return r
+
+
+class BatchGraph:
+    """A resident batch replayed from a captured HIP graph (mh_batch_graph_*):
+    create once for a list of circuits and their terms, update() with new
+    parameters for the same circuits, run() to get the terms."""
+
+    def __init__(self, circuits, terms, *, precision=C128, fusion=FUSE_CONTAINMENT, fusion_width=0, host_threads=0,
+                 max_pass_cost=0, group_bytes=0, kernel_streams=0, cached_states=False):
+        self._keeps = [_CircuitKeep(c) for c in circuits]
+        self._arr = (CCircuit * max(len(self._keeps), 1))(*[k.c for k in self._keeps])
+        self.offs = np.zeros(len(circuits) + 1, np.uint32)
+        for i, t in enumerate(terms):
+            self.offs[i + 1] = self.offs[i] + len(t)
+        self._xs = np.array([x for t in terms for x, _ in t] or [0], np.uint64)
+        self._zs = np.array([z for t in terms for _, z in t] or [0], np.uint64)
+        self._phases = np.array([c.global_phase for c in circuits] or [0.0], np.float64)
+        req = CBatchRequest(len(circuits), self._arr, precision, fusion, fusion_width, _ptr(self._phases, _f64p),
+                            _ptr(self.offs, _u32p), _ptr(self._xs, _u64p), _ptr(self._zs, _u64p), 0, host_threads, 0,
+                            max_pass_cost, group_bytes, kernel_streams, 1 if cached_states else 0)
+        self.ptr = ctypes.c_void_p()
+        check(lib().mh_batch_graph_create(ctypes.byref(req), ctypes.byref(self.ptr)))
+        self.count = len(circuits)
+
+    def update(self, circuits):
+        """New parameters for the same circuits (same structure)."""
+        keeps = [_CircuitKeep(c) for c in circuits]
+        arr = (CCircuit * max(len(keeps), 1))(*[k.c for k in keeps])
+        phases = np.array([c.global_phase for c in circuits] or [0.0], np.float64)
+        check(lib().mh_batch_graph_update(self.ptr, arr, _ptr(phases, _f64p)))
+
+    def run(self):
+        """Replays the graph: (one complex array of terms per circuit, CBatchResult)."""
+        out = np.zeros(2 * max(int(self.offs[-1]), 1), np.float64)
+        res = CBatchResult(_ptr(out, _f64p), None, None)
+        check(lib().mh_batch_graph_run(self.ptr, ctypes.byref(res)))
+        e = out[:2 * int(self.offs[-1])].view(np.complex128)
+        return [e[self.offs[i]:self.offs[i + 1]].copy() for i in range(self.count)], res
+
+    def close(self):
+        if self.ptr:
+            check(lib().mh_batch_graph_destroy(self.ptr))
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class CServerOptions(ctypes.Structure):

@@ -282,6 +282,15 @@ class FlatAnsatz:
         self._params = np.ascontiguousarray(angles, np.float64)
         assert self._params.size == 2 * n * d and self._opcodes.size == per_layer * d
         self.ops = range(self._opcodes.size)  # len() only
+        self.depth = d
+
+    @property
+    def num_angles(self) -> int:
+        return self._params.size
+
+    def with_angles(self, angles) -> "FlatAnsatz":
+        """The same circuit with new angles (a parameter-only update)."""
+        return FlatAnsatz(self.num_qubits, self.depth, angles)
 
     def flat(self):
         return self._opcodes, self._qo, self._qubits, self._po, self._params
