@@ -215,6 +215,10 @@ def served(args, H, W) -> dict:
            "jobs": len(jobs),
            "estimators": sum(s.kind == "estimator" for s in specs),
            "updates": sum(s.update for s in specs)}
+    if args.serve_skip_capacity:
+        if not args.serve_rate:
+            raise SystemExit("--serve-skip-capacity needs --serve-rate")
+        return served_poisson(args, H, jobs_of, gaps, out, args.serve_rate)
     with H.Server(budget_bytes=256_000_000_000, reserve_bytes=256_000_000_000, complex64_pairs=True) as srv:
         w0 = time.perf_counter()
         res = srv.replay(jobs, np.zeros(len(jobs)))
@@ -233,12 +237,52 @@ def served(args, H, W) -> dict:
                                          "expectation": round(st.expectation_seconds, 3),
                                          "sample": round(st.sample_seconds, 3)}}
     log(f"served capacity: {out['capacity']}")
-    rate = args.serve_rate or 0.8 * cap
+    return served_poisson(args, H, jobs_of, gaps, out, args.serve_rate or 0.8 * cap)
+
+
+def replay_with_progress(srv, jobs, arrivals, every=1000) -> list:
+    """mh_server_replay's arrival process driven from Python, for long runs: a
+    submitter thread submits job i at arrivals[i] seconds from the start while
+    this thread waits for the tickets in order, logging every `every` results
+    (mh_server_replay reports nothing until its last job)."""
+    import threading
+    n = len(jobs)
+    tickets = [None] * n
+    cv = threading.Condition()
+    t0 = time.perf_counter()
+
+    def submitter():
+        for i in range(n):
+            delay = t0 + float(arrivals[i]) - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            t = srv.submit(jobs[i])
+            with cv:
+                tickets[i] = t
+                cv.notify_all()
+
+    th = threading.Thread(target=submitter, daemon=True)
+    th.start()
+    results = []
+    for i in range(n):
+        with cv:
+            while tickets[i] is None:
+                cv.wait()
+        results.append(srv.wait(tickets[i]))
+        if (i + 1) % every == 0:
+            log(f"served Poisson: {i + 1} of {n} jobs done at {time.perf_counter() - t0:.1f} s")
+    th.join()
+    return results
+
+
+def served_poisson(args, H, jobs_of, gaps, out, rate) -> dict:
+    """The served jobs arriving as a Poisson process at `rate` jobs/s."""
+    import numpy as np
     arrivals = gaps / rate
     jobs = jobs_of()
     with H.Server(budget_bytes=256_000_000_000, reserve_bytes=256_000_000_000, complex64_pairs=True) as srv:
         w0 = time.perf_counter()
-        res = srv.replay(jobs, arrivals)
+        res = srv.replay(jobs, arrivals) if len(jobs) <= 2000 else replay_with_progress(srv, jobs, arrivals)
         wall = time.perf_counter() - w0
         st = srv.stats()
     failed = [r for r in res if r.status != 0]
@@ -321,6 +365,8 @@ def main() -> int:
     ap.add_argument("--serve-jobs", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=0.0, help="arrivals per second; 0: 0.8 x measured capacity")
     ap.add_argument("--serve-estimator-precision", choices=["c64", "c128"], default="c64")
+    ap.add_argument("--serve-skip-capacity", action="store_true",
+                    help="only the Poisson replay, at --serve-rate (for the 20,000-job configuration)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-c64-report", action="store_true",
                     help="skip the complex64 form at --c64-qubits and its error against complex128")

@@ -208,7 +208,21 @@ typedef struct mh_circuit {
  * width-capped fusion (fusion_width
  * operands at most, 2..6), order-preserving; the oracle applies the same
  * block list. */
-typedef enum mh_fusion { MH_FUSE_NONE = 0, MH_FUSE_CONTAINMENT = 1, MH_FUSE_WIDTH = 2, MH_FUSE_PAIRS = 3 } mh_fusion;
+typedef enum mh_fusion {
+    MH_FUSE_NONE = 0,
+    MH_FUSE_CONTAINMENT = 1,
+    MH_FUSE_WIDTH = 2,
+    MH_FUSE_PAIRS = 3,
+    /* The engine's autotuner: among the containment plan and width-capped
+     * fusion at widths 2..6 it takes the plan whose gate passes the calibrated
+     * time model predicts fastest for this state size and precision (window
+     * passes at the measured pass rate; a wider block's own pass at the larger
+     * of its HBM time and its arithmetic at the measured FP64, FP32 or
+     * matrix-core rate); ties go to the containment plan. The result and the
+     * estimate report the plan taken (fusion_used, fusion_width_used). No
+     * reference counterpart (the reference always uses its containment rule). */
+    MH_FUSE_AUTO = 4
+} mh_fusion;
 
 /* The execution monitor (reference ExecutionMonitor::at_safe_boundary,
  * include/menos/simulation_engine.h:54-116, which apply_blocks calls after
@@ -266,6 +280,8 @@ typedef struct mh_execution_result {
     double gate_seconds;           /* device time of the gate passes */
     double sample_seconds;
     double expectation_seconds;
+    int32_t fusion_used;           /* the plan run: the request's, or the autotuner's choice */
+    uint32_t fusion_width_used;
 } mh_execution_result;
 
 typedef struct mh_cost_estimate {
@@ -280,6 +296,8 @@ typedef struct mh_cost_estimate {
     uint64_t h2d_bytes;
     uint64_t d2h_bytes;
     double seconds;          /* the calibrated time model's prediction */
+    int32_t fusion_used;     /* the plan costed: the request's, or the autotuner's choice */
+    uint32_t fusion_width_used;
 } mh_cost_estimate;
 
 /* Never fails on an infeasible request: it reports feasible = 0 and a reason
@@ -313,6 +331,7 @@ typedef struct mh_calibration {
     double h2d_bytes_per_second;     /* pinned */
     double d2h_bytes_per_second;     /* pinned */
     double fp64_ops_per_second;      /* non-fused DMUL/DADD */
+    double mfma_flops_per_second;    /* complex64 6-qubit blocks on the matrix cores (v_mfma_f32_32x32x2_f32) */
 } mh_calibration;
 
 /* Measures the device (about a second) and installs the result as the time

@@ -45,16 +45,10 @@ struct RoundStats {
 // Device bytes one state of a round takes in the round's arena (a multiple of 256).
 uint64_t round_state_bytes(uint32_t n, int32_t precision);
 
-// Initialises a round's arena on the main stream with kernels only (no memset,
-// no copy: capturable): every byte zero, then state i's first amplitude
-// (at amplitude offsets[i] from the arena) to (phases[2i], phases[2i+1]).
-// offsets and phases are device memory; bytes is a multiple of 16.
-void launch_round_init(int32_t precision, void* arena, uint64_t bytes, const uint64_t* offsets, const double* phases,
-                       uint32_t count);
-
-// Runs one round: every item's state in one pooled arena, initialised to its
-// global phase, pass p of every item in one launch, all expectation terms in
-// two launches, then each sampling item's draw and statevector copy.
+// Runs one round: every item's state in one pooled arena (no initialisation
+// pass: each state's first window pass generates phase * |0...0>), pass p of
+// every item in one launch, all expectation terms in two launches, then each
+// sampling item's draw and statevector copy.
 // Synchronises. Every item needs window_width(precision) to 30 qubits.
 // `arena` (device memory of at least the items' round_state_bytes in total)
 // holds the states; null takes an arena from the pool for the call.

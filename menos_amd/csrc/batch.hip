@@ -33,25 +33,6 @@
 namespace mh {
 namespace {
 
-constexpr int kThreads = 256;
-
-template <typename V>
-__global__ void __launch_bounds__(kThreads) first_amplitude_kernel(V* base, const uint64_t* offsets,
-                                                                   const double* phases, uint32_t count) {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= count) return;
-    V v;
-    v.x = static_cast<decltype(v.x)>(phases[2 * i]);
-    v.y = static_cast<decltype(v.y)>(phases[2 * i + 1]);
-    base[offsets[i]] = v;
-}
-
-// Zeros `words` 16-byte words (a fill kernel rather than a memset node, so
-// that it can sit in a captured graph).
-__global__ void __launch_bounds__(kThreads) zero_kernel(uint4* p, uint64_t words) {
-    const uint64_t stride = uint64_t(gridDim.x) * kThreads;
-    for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < words; i += stride) p[i] = make_uint4(0, 0, 0, 0);
-}
 
 struct Planned {
     uint32_t n = 0;
@@ -82,22 +63,6 @@ void parallel_for(std::size_t count, uint32_t threads, const std::function<void(
     if (error) std::rethrow_exception(error);
 }
 
-void launch_round_init(int32_t precision, void* arena, uint64_t bytes, const uint64_t* offsets, const double* phases,
-                       uint32_t count) {
-    const uint64_t words = bytes / 16;
-    const uint32_t zgrid = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(1, (words + kThreads - 1) / kThreads), 8192));
-    hipLaunchKernelGGL(zero_kernel, dim3(zgrid), dim3(kThreads), 0, streams().main, static_cast<uint4*>(arena), words);
-    MH_HIP(hipGetLastError());
-    const dim3 grid((count + kThreads - 1) / kThreads), block(kThreads);
-    if (precision == MH_C128)
-        hipLaunchKernelGGL(first_amplitude_kernel<double2>, grid, block, 0, streams().main, static_cast<double2*>(arena),
-                           offsets, phases, count);
-    else
-        hipLaunchKernelGGL(first_amplitude_kernel<float2>, grid, block, 0, streams().main, static_cast<float2*>(arena),
-                           offsets, phases, count);
-    MH_HIP(hipGetLastError());
-}
-
 uint64_t round_state_bytes(uint32_t n, int32_t precision) {
     return ((uint64_t{1} << n) * amp_bytes(precision) + kAlign - 1) / kAlign * kAlign;
 }
@@ -119,20 +84,18 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
     DeviceBuffer own(arena_in ? 0 : bytes);
     unsigned char* const arena = static_cast<unsigned char*>(arena_in ? arena_in : own.get());
     const std::size_t count = items.size();
-    std::vector<uint64_t> offsets(count);  // amplitudes from the arena start
-    std::vector<double> phases(2 * count);
     std::vector<BatchItem> passes(count);
     std::vector<ExpectationItem> exp_items;
     uint64_t at = 0;
     for (std::size_t k = 0; k < count; ++k) {
         const RoundItem& it = items[k];
-        offsets[k] = at / amp;
         passes[k].state = arena + at;
         passes[k].n = it.n;
         passes[k].max_cost = it.max_pass_cost;
         const std::complex<double> ph = host_phase(it.global_phase);
-        phases[2 * k] = ph.real();
-        phases[2 * k + 1] = ph.imag();
+        passes[k].initialise = true;
+        passes[k].phase_re = ph.real();
+        passes[k].phase_im = ph.imag();
         if (it.num_terms > 0) {
             ExpectationItem e;
             e.state = passes[k].state;
@@ -145,9 +108,6 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
         }
         at += round_state_bytes(it.n, precision);
     }
-    // Initialise: zeros, then each state's first amplitude.
-    MH_HIP(hipMemsetAsync(arena, 0, bytes, streams().main));
-    DeviceBuffer doffsets(offsets.size() * sizeof(uint64_t)), dphases(phases.size() * sizeof(double));
     // On an error past this point, wait for the queued work before the
     // buffers it uses go back to the pool.
     struct WaitOnUnwind {
@@ -157,19 +117,9 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
             if (std::uncaught_exceptions() > pending) (void)hipStreamSynchronize(stream);
         }
     } wait_on_unwind{streams().main};
-    upload(doffsets.get(), offsets.data(), offsets.size() * sizeof(uint64_t));
-    upload(dphases.get(), phases.data(), phases.size() * sizeof(double));
-    const uint32_t n_here = static_cast<uint32_t>(count);
-    const dim3 grid((n_here + kThreads - 1) / kThreads), block(kThreads);
-    if (precision == MH_C128)
-        hipLaunchKernelGGL(first_amplitude_kernel<double2>, grid, block, 0, streams().main, reinterpret_cast<double2*>(arena),
-                           static_cast<const uint64_t*>(doffsets.as<uint64_t>()),
-                           static_cast<const double*>(dphases.as<double>()), n_here);
-    else
-        hipLaunchKernelGGL(first_amplitude_kernel<float2>, grid, block, 0, streams().main, reinterpret_cast<float2*>(arena),
-                           static_cast<const uint64_t*>(doffsets.as<uint64_t>()),
-                           static_cast<const double*>(dphases.as<double>()), n_here);
-    MH_HIP(hipGetLastError());
+    // No initialisation pass: each state's first window pass generates
+    // phase * |0...0> instead of loading it (window_body), and the rare state
+    // whose first pass is not a window pass gets an initialisation kernel.
     // Apply, reduce, sample, copy back.
     using clock = std::chrono::steady_clock;
     const auto p0 = clock::now();
@@ -201,7 +151,7 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
         runner.finish();
     }
     const auto p1 = clock::now();
-    uint32_t launches = 2 + bs.launches;
+    uint32_t launches = bs.launches;
     if (!exp_items.empty()) {
         expectation_batch(precision, exp_items, terms_out);
         synchronize();

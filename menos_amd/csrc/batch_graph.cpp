@@ -8,11 +8,11 @@
 // every device table and the term-reduction descriptors; its device work is
 // captured once into a graph:
 //
-//   matrix table <- pinned staging (copy node)
-//   phases       <- pinned staging (copy node)
-//   arena zeroed, first amplitudes written (kernels: a graph holds no memset)
+//   matrix table <- pinned staging (copy node; it also holds each circuit's
+//                   initial phase)
 //   every pass of every circuit (the batch program's launches, forked over the
-//   kernel streams and joined back)
+//   kernel streams and joined back; each circuit's first pass generates its
+//   initial state, so the arena needs no clearing)
 //   term reductions (kernels)
 //   terms -> pinned staging (copy node)
 //
@@ -126,9 +126,9 @@ struct mh_batch_graph {
     std::unique_ptr<mh::ExpectationProgram> expectation;
     std::vector<uint32_t> term_offsets;
     // Device residency.
-    mh::DeviceBuffer arena, bundles, ops, terms, mats, args, groups, offsets, phases, workspace;
+    mh::DeviceBuffer arena, bundles, ops, terms, mats, args, groups, workspace;
     uint64_t arena_bytes = 0;
-    mh::Pinned pinned_mats, pinned_phases, pinned_terms;
+    mh::Pinned pinned_mats, pinned_terms;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<hipEvent_t> events;
@@ -162,11 +162,13 @@ void plan_all(mh_batch_graph& g, const mh_circuit* circuits) {
     g.plan_seconds = seconds_since(t0);
 }
 
-void phases_into(const mh_batch_graph& g, const double* global_phases, double* out) {
-    for (uint32_t i = 0; i < g.count; ++i) {
+// Each item's initial phase exp(i * global_phase), from host libm.
+void set_phases(std::vector<BatchItem>& items, const double* global_phases) {
+    for (std::size_t i = 0; i < items.size(); ++i) {
         const std::complex<double> ph = host_phase(global_phases ? global_phases[i] : 0.0);
-        out[2 * i] = ph.real();
-        out[2 * i + 1] = ph.imag();
+        items[i].initialise = true;
+        items[i].phase_re = ph.real();
+        items[i].phase_im = ph.imag();
     }
 }
 
@@ -187,14 +189,10 @@ void capture(mh_batch_graph& g) {
     MH_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
     try {
         MH_HIP(hipMemcpyAsync(g.mats.get(), g.pinned_mats.get(), g.program.mats().size(), hipMemcpyHostToDevice, main));
-        MH_HIP(hipMemcpyAsync(g.phases.get(), g.pinned_phases.get(), 2 * sizeof(double) * g.count,
-                              hipMemcpyHostToDevice, main));
-        launch_round_init(g.precision, g.arena.get(), g.arena_bytes, g.offsets.as<uint64_t>(), g.phases.as<double>(),
-                          g.count);
         // Fork the kernel streams off the main stream, run the program, join.
         MH_HIP(hipEventRecord(g.events[0], main));
         for (uint32_t k = 1; k < nstreams; ++k) MH_HIP(hipStreamWaitEvent(st.kernel(k), g.events[0], 0));
-        g.launches = 2 + issue_batch_program(g.program, g.items.data(), dev, g.schedule, 0);
+        g.launches = issue_batch_program(g.program, g.items.data(), dev, g.schedule, 0);
         for (uint32_t k = 1; k < nstreams; ++k) {
             MH_HIP(hipEventRecord(g.events[Streams::kKernelStreams + k], st.kernel(k)));
             MH_HIP(hipStreamWaitEvent(main, g.events[Streams::kKernelStreams + k], 0));
@@ -226,7 +224,7 @@ int mh_batch_graph_create(const mh_batch_request* req, mh_batch_graph** out) {
     return guarded([&] {
         if (req == nullptr || out == nullptr) fail(MH_ERR_INVALID, "null request or output");
         *out = nullptr;
-        const std::size_t amp = amp_bytes(req->precision);
+        amp_bytes(req->precision);
         if (req->num_circuits == 0 || req->circuits == nullptr || req->term_offsets == nullptr)
             fail(MH_ERR_INVALID, "a captured batch needs circuits and term offsets");
         if (req->want_statevectors) fail(MH_ERR_INVALID, "a captured batch returns terms only");
@@ -272,7 +270,6 @@ int mh_batch_graph_create(const mh_batch_request* req, mh_batch_graph** out) {
         g->arena_bytes = arena_bytes;
         g->arena = DeviceBuffer(arena_bytes);
         unsigned char* base = static_cast<unsigned char*>(g->arena.get());
-        std::vector<uint64_t> offsets(g->count);
         std::vector<ExpectationItem> exp_items;
         uint64_t at = 0;
         for (uint32_t i = 0; i < g->count; ++i) {
@@ -281,7 +278,6 @@ int mh_batch_graph_create(const mh_batch_request* req, mh_batch_graph** out) {
             it.n = req->circuits[i].num_qubits;
             it.blocks = &g->blocks[i];
             it.max_cost = req->max_pass_cost;
-            offsets[i] = at / amp;
             const uint32_t nt = req->term_offsets[i + 1] - req->term_offsets[i];
             if (nt > 0) {
                 ExpectationItem e;
@@ -295,11 +291,13 @@ int mh_batch_graph_create(const mh_batch_request* req, mh_batch_graph** out) {
             }
             at += round_state_bytes(it.n, req->precision);
         }
+        set_phases(g->items, req->global_phases);
         const auto c0 = std::chrono::steady_clock::now();
         g->program = compile_batch_program(req->precision, g->items.data(), g->count, g->threads, g->schedule);
         g->compile_seconds = seconds_since(c0);
-        if (g->program.has_generic())
-            fail(MH_ERR_INVALID, "a captured batch takes blocks of at most 2 qubits (dense) or diagonal blocks");
+        if (g->program.has_generic() || g->program.needs_initialisation())
+            fail(MH_ERR_INVALID, "a captured batch takes circuits with at least one gate and blocks of at most 2 "
+                                 "qubits (dense) or diagonal blocks");
         auto table = [](DeviceBuffer& d, const std::vector<unsigned char>& src) {
             d = DeviceBuffer(src.size());
             upload(d.get(), src.data(), src.size());
@@ -310,13 +308,8 @@ int mh_batch_graph_create(const mh_batch_request* req, mh_batch_graph** out) {
         table(g->args, g->program.args());
         table(g->groups, g->program.groups());
         g->mats = DeviceBuffer(g->program.mats().size());
-        table(g->offsets, std::vector<unsigned char>(reinterpret_cast<unsigned char*>(offsets.data()),
-                                                     reinterpret_cast<unsigned char*>(offsets.data() + g->count)));
-        g->phases = DeviceBuffer(2 * sizeof(double) * g->count);
         g->pinned_mats = Pinned(g->program.mats().size());
         std::memcpy(g->pinned_mats.get(), g->program.mats().data(), g->program.mats().size());
-        g->pinned_phases = Pinned(2 * sizeof(double) * g->count);
-        phases_into(*g, req->global_phases, static_cast<double*>(g->pinned_phases.get()));
         if (!exp_items.empty()) {
             g->expectation = std::make_unique<ExpectationProgram>(req->precision, exp_items);
             g->workspace = DeviceBuffer(g->expectation->workspace_bytes());
@@ -346,6 +339,7 @@ int mh_batch_graph_update(mh_batch_graph* g, const mh_circuit* circuits, const d
         g->plan_seconds = seconds_since(t0);
         std::vector<BatchItem> items = g->items;
         for (uint32_t i = 0; i < g->count; ++i) items[i].blocks = &fresh[i];
+        set_phases(items, global_phases);
         const auto c0 = clock_type::now();
         BatchProgram prog = compile_batch_program(g->precision, items.data(), g->count, g->threads, g->schedule);
         g->compile_seconds = seconds_since(c0);
@@ -353,8 +347,8 @@ int mh_batch_graph_update(mh_batch_graph* g, const mh_circuit* circuits, const d
             fail(MH_ERR_INVALID, "the new parameters change a pass program (a matrix entry became or stopped being "
                                  "exactly zero or +-1); create a new captured batch for them");
         std::memcpy(g->pinned_mats.get(), prog.mats().data(), prog.mats().size());
-        phases_into(*g, global_phases, static_cast<double*>(g->pinned_phases.get()));
         g->blocks = std::move(fresh);
+        g->items = std::move(items);
         for (uint32_t i = 0; i < g->count; ++i) g->items[i].blocks = &g->blocks[i];
         g->program = std::move(prog);
     });

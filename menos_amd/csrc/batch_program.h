@@ -66,6 +66,9 @@ public:
     // sign terms and launch lists (its matrix table alone may differ).
     bool same_shape(const BatchProgram& other) const;
     bool has_generic() const;
+    // Whether some state needs an initialisation kernel (its first pass is not
+    // a window pass that can generate it).
+    bool needs_initialisation() const;
     uint32_t launch_count() const;
 
     BatchProgramData& data();

@@ -12,6 +12,7 @@
 #include <cmath>
 
 #include "mh_internal.h"
+#include "window.h"
 
 namespace mh {
 
@@ -140,6 +141,18 @@ extern "C" int mh_calibrate(mh_calibration* out) {
                                    1.0000001, 1e-9);
             });
             c.fp64_ops_per_second = static_cast<double>(grid) * kThreads * iters * 16.0 / s;
+        }
+        {
+            // A complex64 6-qubit block on the matrix cores at 26 qubits
+            // (arithmetic-bound): 8 * 64 flops per amplitude.
+            const uint32_t n = 26, k = 6;
+            DeviceBuffer st((uint64_t{1} << n) * 8);
+            launch_init(st.get(), n, MH_C64, 1.0, 0.0);
+            std::vector<cplx> m(std::size_t{1} << (2 * k));
+            for (std::size_t e = 0; e < m.size(); ++e) m[e] = cplx{e % 65 == 0 ? 1.0 : 0.0, 0.0};
+            const uint32_t ops[] = {3, 7, 11, 15, 19, 23};
+            const double s = best_seconds(3, [&] { launch_generic(st.get(), n, MH_C64, m.data(), ops, k); });
+            c.mfma_flops_per_second = 8.0 * 64.0 * static_cast<double>(uint64_t{1} << n) / s;
         }
         synchronize();
         set_calibration(c);

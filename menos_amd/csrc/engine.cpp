@@ -10,6 +10,7 @@
 // sampler.hip) instead of a copy-back and a host walk.
 #include "engine.hpp"
 #include "gate_table.h"
+#include "window.h"
 
 #include <algorithm>
 #include <chrono>
@@ -55,26 +56,83 @@ uint64_t scratch_bytes(uint32_t n, uint64_t shots, std::size_t terms) {
 // what a streaming kernel reaches; mh_calibrate() replaces them by measurement.
 mh_calibration g_calibration = {
     /*hbm_bytes_per_second=*/6.0e12, /*pass_bytes_per_second=*/4.0e12, /*launch_seconds=*/8e-6,
-    /*h2d_bytes_per_second=*/50e9, /*d2h_bytes_per_second=*/50e9, /*fp64_ops_per_second=*/30e12};
+    /*h2d_bytes_per_second=*/50e9, /*d2h_bytes_per_second=*/50e9, /*fp64_ops_per_second=*/30e12,
+    /*mfma_flops_per_second=*/100e12};
 
 double word_to_uniform(uint64_t w) { return static_cast<double>(w >> 11) * (1.0 / 9007199254740992.0); }
 
 struct Plan {
     FusionPlan fusion;
     std::vector<Block> blocks;
+    int32_t fusion_used = MH_FUSE_CONTAINMENT;
+    uint32_t width_used = 0;
 };
+
+uint32_t window_for(const ExecutionRequest& r) {
+    return r.window_qubits == 0 ? window_default(r.precision) : r.window_qubits;
+}
+
+// The calibrated time model's device seconds for the gate passes of a block
+// list (launch costs apart): a window pass moves the state in and out at the measured pass rate; a
+// block on a pass of its own (the per-group kernel, or the matrix cores for
+// complex64 blocks of 5-6 qubits) takes the larger of its HBM time and its
+// arithmetic (8 * 2^k operations per amplitude) at the measured FP64 rate
+// (complex128, no FMA), twice that (complex64 FMA) or the matrix-core rate.
+double pass_seconds(uint32_t n, int32_t precision, const std::vector<Block>& blocks, uint32_t w, uint32_t max_cost) {
+    const mh_calibration& c = g_calibration;
+    const double sb = static_cast<double>(state_bytes(n, precision));
+    const double amps = static_cast<double>(uint64_t{1} << n);
+    double seconds = 0;
+    for (const WindowPass& pass : plan_window_passes(n, precision, blocks, w, max_cost == 0 ? kDefaultPassCost : max_cost)) {
+        if (!pass.generic) {
+            seconds += 2.0 * sb / c.pass_bytes_per_second;
+            continue;
+        }
+        const uint32_t k = static_cast<uint32_t>(blocks[pass.blocks[0]].operands.size());
+        const double ops = 8.0 * static_cast<double>(1u << k) * amps;
+        const double rate = precision == MH_C128 ? c.fp64_ops_per_second
+                            : (precision == MH_C64 && k >= 5 && n >= k + 5) ? c.mfma_flops_per_second
+                                                                           : 2.0 * c.fp64_ops_per_second;
+        seconds += std::max(2.0 * sb / c.hbm_bytes_per_second, ops / rate);
+    }
+    return seconds;
+}
 
 Plan make_plan(const ExecutionRequest& r) {
     validate_circuit(r.circuit);
     const CircuitView cv{r.circuit};
     Plan p;
-    p.fusion = plan_fusion(cv, r.fusion, r.fusion_width);
-    p.blocks = build_blocks(cv, p.fusion);
+    if (r.fusion != MH_FUSE_AUTO) {
+        p.fusion = plan_fusion(cv, r.fusion, r.fusion_width);
+        p.blocks = build_blocks(cv, p.fusion);
+        p.fusion_used = r.fusion;
+        p.width_used = r.fusion == MH_FUSE_WIDTH ? r.fusion_width : 0;
+        return p;
+    }
+    // The autotuner: containment first, then widths 2..6; strictly faster wins.
+    const uint32_t n = r.circuit->num_qubits;
+    double best = 0;
+    bool have = false;
+    for (uint32_t cand = 0; cand <= 5; ++cand) {
+        const int32_t fusion = cand == 0 ? MH_FUSE_CONTAINMENT : MH_FUSE_WIDTH;
+        const uint32_t width = cand == 0 ? 0 : cand + 1;
+        Plan q;
+        q.fusion = plan_fusion(cv, fusion, width);
+        q.blocks = build_blocks(cv, q.fusion);
+        q.fusion_used = fusion;
+        q.width_used = width;
+        const double t = n < window_default(r.precision)
+                             ? static_cast<double>(q.blocks.size())
+                             : pass_seconds(n, r.precision, q.blocks, window_for(r), r.max_pass_cost) +
+                                   g_calibration.launch_seconds *
+                                       count_passes(n, r.precision, q.blocks, window_for(r), r.max_pass_cost);
+        if (!have || t < best) {
+            best = t;
+            p = std::move(q);
+            have = true;
+        }
+    }
     return p;
-}
-
-uint32_t window_for(const ExecutionRequest& r) {
-    return r.window_qubits == 0 ? window_default(r.precision) : r.window_qubits;
 }
 
 }  // namespace
@@ -89,6 +147,8 @@ CostEstimate HipDenseEngine::estimate(const ExecutionRequest& r) const noexcept 
         const Plan p = make_plan(r);
         e.blocks = static_cast<uint32_t>(p.blocks.size());
         e.passes = count_passes(n, r.precision, p.blocks, window_for(r), r.max_pass_cost);
+        e.fusion_used = p.fusion_used;
+        e.width_used = p.width_used;
         const uint64_t sb = state_bytes(n, r.precision);
         e.device_bytes = sb + scratch_bytes(n, r.shots, r.terms.size());
         e.host_bytes = r.want_statevector ? sb : 0;
@@ -104,7 +164,9 @@ CostEstimate HipDenseEngine::estimate(const ExecutionRequest& r) const noexcept 
         e.d2h_bytes = (r.want_statevector ? sb : 0) + r.shots * 8 + r.terms.size() * 16;
         const mh_calibration& c = g_calibration;
         double seconds = static_cast<double>(e.launches) * c.launch_seconds;
-        seconds += static_cast<double>(e.passes) * 2.0 * static_cast<double>(sb) / c.pass_bytes_per_second;
+        seconds += n < window_default(r.precision)
+                       ? static_cast<double>(e.passes) * 2.0 * static_cast<double>(sb) / c.pass_bytes_per_second
+                       : pass_seconds(n, r.precision, p.blocks, window_for(r), r.max_pass_cost);
         seconds += static_cast<double>(sb) / c.hbm_bytes_per_second;  // initialisation
         seconds += (r.shots ? 3.0 : 0.0) * static_cast<double>(sb) / c.hbm_bytes_per_second;
         seconds += static_cast<double>(r.terms.size()) * static_cast<double>(sb) / c.hbm_bytes_per_second;
@@ -161,6 +223,8 @@ ExecutionResult HipDenseEngine::execute(const ExecutionRequest& r) const {
 
     ExecutionResult result;
     result.num_blocks = static_cast<uint32_t>(p.blocks.size());
+    result.fusion_used = p.fusion_used;
+    result.width_used = p.width_used;
     result.fused_blocks = p.fusion.fused_blocks;
     result.gates_absorbed = p.fusion.gates_absorbed;
     {
@@ -281,6 +345,8 @@ int mh_engine_estimate(const mh_execution_request* request, mh_cost_estimate* ou
         out->h2d_bytes = e.h2d_bytes;
         out->d2h_bytes = e.d2h_bytes;
         out->seconds = e.seconds;
+        out->fusion_used = e.fusion_used;
+        out->fusion_width_used = e.width_used;
     });
 }
 
@@ -308,6 +374,8 @@ int mh_engine_execute(const mh_execution_request* request, mh_execution_result* 
         }
         out->num_blocks = res.num_blocks;
         out->num_passes = res.num_passes;
+        out->fusion_used = res.fusion_used;
+        out->fusion_width_used = res.width_used;
         out->fused_blocks = res.fused_blocks;
         out->gates_absorbed = res.gates_absorbed;
         out->actual_bytes = res.actual_bytes;
@@ -332,7 +400,7 @@ int mh_set_calibration(const mh_calibration* profile) {
         if (profile == nullptr) fail(MH_ERR_INVALID, "null profile");
         const double rates[] = {profile->hbm_bytes_per_second, profile->pass_bytes_per_second,
                                 profile->h2d_bytes_per_second, profile->d2h_bytes_per_second,
-                                profile->fp64_ops_per_second};
+                                profile->fp64_ops_per_second, profile->mfma_flops_per_second};
         for (double r : rates)
             if (!(r > 0.0) || !std::isfinite(r)) fail(MH_ERR_INVALID, "profile rates must be finite and positive");
         if (!(profile->launch_seconds >= 0.0) || !std::isfinite(profile->launch_seconds))

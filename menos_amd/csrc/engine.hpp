@@ -81,6 +81,8 @@ struct ExecutionResult {
     uint64_t fused_blocks = 0, gates_absorbed = 0;
     std::vector<float> pass_ms;
     double execute_seconds = 0, gate_seconds = 0, sample_seconds = 0, expectation_seconds = 0;
+    int32_t fusion_used = MH_FUSE_CONTAINMENT;
+    uint32_t width_used = 0;
 };
 
 struct CostEstimate {
@@ -90,6 +92,8 @@ struct CostEstimate {
     uint32_t blocks = 0, passes = 0;
     uint64_t launches = 0, h2d_bytes = 0, d2h_bytes = 0;
     double seconds = 0;
+    int32_t fusion_used = MH_FUSE_CONTAINMENT;
+    uint32_t width_used = 0;
 };
 
 // Draws `shots` outcomes from a device state and folds them into counts keyed

@@ -88,6 +88,141 @@ __global__ void __launch_bounds__(kThreads) generic_gate_kernel(GenericArgs a) {
         generic_group<R, K>(a, m, g);
 }
 
+// ---- complex64 blocks of K = 4..6 qubits on the matrix cores
+//
+// A block applied to 2^(n-K) groups of 2^K amplitudes is one product
+// OUT = A x IN of real matrices: IN's column j is group j's amplitudes as
+// 2^(K+1) reals (re, im of amplitude 0, re, im of amplitude 1, ...), and A is
+// the complex matrix M in the same real form, A[2o + p][2m + q] =
+//   Mr[o][m] (p = q = 0), -Mi[o][m] (p = 0, q = 1), Mi[o][m] (p = 1, q = 0),
+//   Mr[o][m] (p = q = 1).
+// A wave takes 32 groups at a time (a tile): per row block rb of 32 rows of
+// A and per amplitude s of the group, one v_mfma_f32_32x32x2_f32 adds
+// A[rows][2s, 2s + 1] x IN[2s, 2s + 1][32 groups]. Operand layouts (wave64):
+// lane l holds A[32 rb + l % 32][2s + l / 32] and IN[2s + l / 32][l % 32];
+// output register r of lane l is row 8 (r / 4) + 4 (l / 32) + r % 4 of the
+// block, column l % 32. Rows 2o and 2o + 1 (re and im of amplitude o) land in
+// consecutive registers of one lane, so each lane stores whole amplitudes.
+// The host lays A out per (rb, s) as the 64 lanes read it. K <= 5 keeps A in
+// registers across tiles; K = 6 (A is 64 KiB) stages it in LDS.
+constexpr uint32_t kMfmaTileGroups = 32;
+
+typedef float MfmaAcc __attribute__((ext_vector_type(16)));
+
+struct MfmaArgs {
+    float2* state;
+    const float* a;          // (rb * 2^K + s) * 64 + lane
+    uint64_t tiles;          // groups / 32
+    uint32_t ops[kMaxK];     // operand order
+    uint32_t sorted[kMaxK];  // ascending
+};
+
+// One tile of 32 groups; a_at(e) is the lane's A entry e = rb * 2^K + s.
+template <uint32_t K, typename AEntry>
+__device__ __forceinline__ void mfma_tile(const MfmaArgs& a, const AEntry& a_at, uint64_t tile, uint32_t lane) {
+    constexpr uint32_t S = 1u << K;               // amplitudes per group
+    constexpr uint32_t RBLK = (2u * S) / 32u;      // row blocks of A
+    const uint32_t j = lane & 31u, h = lane >> 5;
+    const uint64_t base = group_base(tile * kMfmaTileGroups + j, a.sorted, K);
+    const float* st = reinterpret_cast<const float*>(a.state);
+    float in[S];
+#pragma unroll
+    for (uint32_t s = 0; s < S; ++s) in[s] = st[2 * group_member(base, s, a.ops, K) + h];
+    MfmaAcc acc[RBLK];
+#pragma unroll
+    for (uint32_t rb = 0; rb < RBLK; ++rb) {
+        acc[rb] = MfmaAcc{};
+#pragma unroll
+        for (uint32_t s = 0; s < S; ++s)
+            acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a_at(rb * S + s), in[s], acc[rb], 0, 0, 0);
+    }
+#pragma unroll
+    for (uint32_t rb = 0; rb < RBLK; ++rb) {
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+            const uint32_t o = 16 * rb + 4 * q + 2 * h;
+            a.state[group_member(base, o, a.ops, K)] = make_float2(acc[rb][4 * q], acc[rb][4 * q + 1]);
+            a.state[group_member(base, o + 1, a.ops, K)] = make_float2(acc[rb][4 * q + 2], acc[rb][4 * q + 3]);
+        }
+    }
+}
+
+// Waves take tiles wave, wave + waves, ... (every wave exits when the tiles
+// run out).
+template <uint32_t K>
+__global__ void __launch_bounds__(kThreads) mfma_gate_kernel(MfmaArgs a) {
+    constexpr uint32_t S = 1u << K;
+    constexpr uint32_t A_FLOATS = (2u * S / 32u) * S * 64u;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t wave = uint64_t(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6);
+    const uint64_t waves = uint64_t(gridDim.x) * (kThreads / 64);
+    if constexpr (K == 6) {
+        // A (64 KiB) from LDS, staged once per workgroup; every thread reaches
+        // the barrier before any tile.
+        __shared__ float am[A_FLOATS];
+        for (uint32_t e = threadIdx.x; e < A_FLOATS; e += kThreads) am[e] = a.a[e];
+        __syncthreads();
+        const auto a_at = [&](uint32_t e) { return am[e * 64 + lane]; };
+        for (uint64_t t = wave; t < a.tiles; t += waves) mfma_tile<K>(a, a_at, t, lane);
+    } else {
+        // A in registers: the lane's own entries, loaded once.
+        float am[A_FLOATS / 64];
+#pragma unroll
+        for (uint32_t e = 0; e < A_FLOATS / 64; ++e) am[e] = a.a[e * 64 + lane];
+        const auto a_at = [&](uint32_t e) { return am[e]; };
+        for (uint64_t t = wave; t < a.tiles; t += waves) mfma_tile<K>(a, a_at, t, lane);
+    }
+}
+
+// A's real form in the lane order the kernel reads (see above).
+std::vector<float> mfma_matrix(const cplx* m, uint32_t k) {
+    const uint32_t S = 1u << k, D = 2u * S, rblk = D / 32u;
+    std::vector<float> out(std::size_t(rblk) * S * 64);
+    for (uint32_t rb = 0; rb < rblk; ++rb)
+        for (uint32_t s = 0; s < S; ++s)
+            for (uint32_t l = 0; l < 64; ++l) {
+                const uint32_t row = 32 * rb + (l & 31u), col = 2 * s + (l >> 5);
+                const uint32_t o = row >> 1, p = row & 1u, mm = col >> 1, q = col & 1u;
+                const cplx e = m[std::size_t(o) * S + mm];
+                const double v = p == q ? e.real() : (p == 0 ? -e.imag() : e.imag());
+                out[(std::size_t(rb) * S + s) * 64 + l] = static_cast<float>(v);
+            }
+    return out;
+}
+
+// Blocks the matrix cores take: complex64, 5 or 6 qubits, at least one tile
+// of 32 groups.
+bool mfma_applies(int32_t precision, uint32_t n, uint32_t k) {
+    return precision == MH_C64 && k >= 5 && k <= 6 && n >= k + 5;
+}
+
+void run_mfma(void* state, uint32_t n, const cplx* matrix, const uint32_t* operands, uint32_t k) {
+    context();
+    const std::vector<float> am = mfma_matrix(matrix, k);
+    DeviceBuffer dm(am.size() * sizeof(float));
+    upload(dm.get(), am.data(), am.size() * sizeof(float));
+    MfmaArgs a{};
+    a.state = static_cast<float2*>(state);
+    a.a = dm.as<float>();
+    if (k < 4 || k > 6 || n < k + 5) fail(MH_ERR_INTERNAL, "matrix-core kernel: block shape out of range");
+    a.tiles = ((uint64_t{1} << n) >> k) / kMfmaTileGroups;
+    for (uint32_t j = 0; j < k; ++j) a.ops[j] = a.sorted[j] = operands[j];
+    std::sort(a.sorted, a.sorted + k);
+    for (uint32_t j = 0; j < k; ++j)
+        if (a.sorted[j] >= n || (j > 0 && a.sorted[j] == a.sorted[j - 1]))
+            fail(MH_ERR_INTERNAL, "matrix-core kernel: operands out of range or repeated");
+    // Four waves per workgroup, at most one tile per wave per round of the grid.
+    const uint64_t groups_of_waves = (a.tiles + (kThreads / 64) - 1) / (kThreads / 64);
+    const dim3 grid(static_cast<uint32_t>(std::min<uint64_t>(groups_of_waves, 4096))), block(kThreads);
+    switch (k) {
+        case 4: hipLaunchKernelGGL(mfma_gate_kernel<4>, grid, block, 0, streams().main, a); break;
+        case 5: hipLaunchKernelGGL(mfma_gate_kernel<5>, grid, block, 0, streams().main, a); break;
+        default: hipLaunchKernelGGL(mfma_gate_kernel<6>, grid, block, 0, streams().main, a); break;
+    }
+    MH_HIP(hipGetLastError());
+    synchronize();
+}
+
 template <typename R>
 __global__ void __launch_bounds__(kThreads) init_kernel(void* state, uint64_t size, R re, R im) {
     using V = typename Vec<R>::type;
@@ -189,15 +324,19 @@ bool matrix_is_diagonal(const cplx* m, std::size_t dim) {
     return true;
 }
 
-void launch_init(void* state, uint32_t n, int32_t precision, double re, double im) {
+void launch_init_on(void* state, uint32_t n, int32_t precision, double re, double im, hipStream_t stream) {
     context();
     const uint64_t size = uint64_t{1} << n;
     if (precision == MH_C128)
-        hipLaunchKernelGGL(init_kernel<double>, dim3(grid_for(size)), dim3(kThreads), 0, streams().main, state, size, re, im);
+        hipLaunchKernelGGL(init_kernel<double>, dim3(grid_for(size)), dim3(kThreads), 0, stream, state, size, re, im);
     else
-        hipLaunchKernelGGL(init_kernel<float>, dim3(grid_for(size)), dim3(kThreads), 0, streams().main, state, size,
+        hipLaunchKernelGGL(init_kernel<float>, dim3(grid_for(size)), dim3(kThreads), 0, stream, state, size,
                            static_cast<float>(re), static_cast<float>(im));
     MH_HIP(hipGetLastError());
+}
+
+void launch_init(void* state, uint32_t n, int32_t precision, double re, double im) {
+    launch_init_on(state, n, precision, re, im, streams().main);
 }
 
 void apply_blocks(void* state, uint32_t n, int32_t precision, const std::vector<Block>& blocks,
@@ -223,6 +362,8 @@ void launch_generic(void* state, uint32_t n, int32_t precision, const cplx* matr
                     uint32_t k) {
     if (precision == MH_C128)
         run_generic<double>(state, n, matrix, operands, k);
+    else if (mfma_applies(precision, n, k))
+        run_mfma(state, n, matrix, operands, k);
     else
         run_generic<float>(state, n, matrix, operands, k);
 }
@@ -231,7 +372,12 @@ void apply_gate(void* state, uint32_t n, int32_t precision, const cplx* matrix, 
                 bool generic_only) {
     check_operands(n, operands, k);
     if (generic_only) {
-        launch_generic(state, n, precision, matrix, operands, k);
+        // The one-thread-per-group kernel itself, at either precision (the
+        // independent path the others are checked against).
+        if (precision == MH_C128)
+            run_generic<double>(state, n, matrix, operands, k);
+        else
+            run_generic<float>(state, n, matrix, operands, k);
         return;
     }
     Block b;

@@ -301,6 +301,8 @@ FusionPlan plan_fusion(const CircuitView& cv, int32_t fusion, uint32_t width) {
         case MH_FUSE_NONE: return unfused_plan(cv);
         case MH_FUSE_CONTAINMENT: return containment_plan(cv, false);
         case MH_FUSE_PAIRS: return containment_plan(cv, true);
+        case MH_FUSE_AUTO:
+            fail(MH_ERR_INVALID, "MH_FUSE_AUTO is resolved by the engine (mh_engine_estimate reports the plan it takes)");
         case MH_FUSE_WIDTH:
             if (width < 1 || width > 6) fail(MH_ERR_INVALID, "fusion_width must be 1..6");
             return width_plan(cv, width);

@@ -125,6 +125,8 @@ void download(void* host_dst, const void* device_src, std::size_t bytes);
 
 // ---- kernels (gates.hip)
 void launch_init(void* state, uint32_t n, int32_t precision, double re, double im);
+// The same on a given stream.
+void launch_init_on(void* state, uint32_t n, int32_t precision, double re, double im, hipStream_t stream);
 // One gate, dispatched (diagonal / k=1 / k=2 / generic) or generic only.
 void apply_gate(void* state, uint32_t n, int32_t precision, const cplx* matrix, const uint32_t* operands,
                 uint32_t k, bool generic_only);

@@ -43,6 +43,11 @@ struct BatchItem {
     uint32_t n = 0;
     const std::vector<Block>* blocks = nullptr;
     uint32_t max_cost = 0;  // per-pass FP work cap; 0: kBatchPassCost
+    // initialise: the state starts as phase * |0...0> (its memory need not be
+    // initialised): the first pass generates it instead of loading it, or, when
+    // that pass is not a window pass, an initialisation kernel runs first.
+    bool initialise = false;
+    double phase_re = 1.0, phase_im = 0.0;
 };
 
 // How a batch's launches are laid out on the device.
@@ -95,7 +100,9 @@ private:
 void run_window_passes_batch(int32_t precision, const std::vector<BatchItem>& items, uint32_t threads,
                              BatchStats* stats);
 
-// The generic one-thread-per-group kernel for one block (gates.hip); synchronises.
+// A block the window passes do not take, on its own pass (gates.hip): the
+// one-thread-per-group kernel, or the matrix cores for complex64 blocks of 5
+// or 6 qubits; synchronises.
 void launch_generic(void* state, uint32_t n, int32_t precision, const cplx* matrix, const uint32_t* operands,
                     uint32_t k);
 

@@ -170,9 +170,12 @@ struct WPassArgs {
 };
 
 // One state's pass in a batched launch: offsets from the launch's base pointers.
+// generate: the pass starts from the initial state phase * |0...0> (the phase
+// at mats[mat_off + phase_at]) instead of loading the state.
 struct WBatchArgs {
     uint64_t state_off;       // amplitudes
     uint32_t bundle_off, op_off, term_off, mat_off;
+    uint32_t generate, phase_at;
     WLayout lay;
 };
 
@@ -568,11 +571,15 @@ __device__ __forceinline__ void reg_swap_stage(uint64_t c, V* v) {
         if (c & (1ull << Stages::reg_swap(J1, J2, RB))) reg_swap<V, E, J1, J2>(v);
 }
 
-// One wave's window of one pass.
+// One wave's window of one pass. With `initial` set the window starts from
+// the initial state instead of a load: *initial at index 0 (wave 0, lane 0,
+// register 0: every index bit clear) and +0 everywhere else, the values the
+// initialisation kernel would have written.
 template <typename R, bool kPairs, bool kTemporal = false>
 __device__ __forceinline__ void window_body(typename Shape<R>::V* state, const uint64_t* bundles, const WOp* ops,
                                             const SignTerm* terms, const typename Shape<R>::V* mats,
-                                            const WLayout& a, uint32_t wave, uint32_t lane) {
+                                            const WLayout& a, uint32_t wave, uint32_t lane,
+                                            const typename Shape<R>::V* initial = nullptr) {
     using V = typename Shape<R>::V;
     constexpr uint32_t RB = Shape<R>::kRegBits;
     constexpr uint32_t E = 1u << RB;
@@ -580,7 +587,15 @@ __device__ __forceinline__ void window_body(typename Shape<R>::V* state, const u
     V* const st = state + obase;
 
     V v[E];
-    {
+    if (initial != nullptr) {
+        const V first = *initial;
+        const bool origin = wave == 0 && lane == 0;
+#pragma unroll
+        for (uint32_t r = 0; r < E; ++r) {
+            v[r].x = r == 0 && origin ? first.x : R(0);
+            v[r].y = r == 0 && origin ? first.y : R(0);
+        }
+    } else {
         const V* src = st + lane_offset(lane, a.load_lane_q);
         uint64_t rq[RB];
 #pragma unroll
@@ -683,8 +698,9 @@ __global__ void __launch_bounds__(kWavesPerGroup * 64, 4)
     const uint32_t wave =
         __builtin_amdgcn_readfirstlane((b - group_begin[lo]) * kWavesPerGroup + (threadIdx.x >> 6));
     if (wave >= a.lay.num_waves) return;
+    const typename Shape<R>::V* m = mats + a.mat_off;
     window_body<R, kPairs, kTemporal>(state + a.state_off, bundles + a.bundle_off, ops + a.op_off, terms + a.term_off,
-                           mats + a.mat_off, a.lay, wave, threadIdx.x & 63u);
+                                      m, a.lay, wave, threadIdx.x & 63u, a.generate ? m + a.phase_at : nullptr);
 }
 
 // ---- host: passes
@@ -1240,6 +1256,7 @@ struct BatchPassRec {
 
 struct BatchProgramData {
     int32_t precision = MH_C128;
+    std::vector<std::size_t> needs_init;             // states an initialisation kernel starts
     std::vector<unsigned char> bundles, ops, terms, mats, args, groups;
     std::vector<std::vector<BatchPassRec>> passes;   // per state
     std::vector<std::vector<std::size_t>> lgroups;   // launch groups of states
@@ -1267,6 +1284,7 @@ const std::vector<unsigned char>& BatchProgram::mats() const { return d_->mats; 
 const std::vector<unsigned char>& BatchProgram::args() const { return d_->args; }
 const std::vector<unsigned char>& BatchProgram::groups() const { return d_->groups; }
 bool BatchProgram::has_generic() const { return d_->has_generic; }
+bool BatchProgram::needs_initialisation() const { return !d_->needs_init.empty(); }
 uint32_t BatchProgram::launch_count() const { return static_cast<uint32_t>(d_->launches.size()); }
 BatchProgramData& BatchProgram::data() { return *d_; }
 const BatchProgramData& BatchProgram::data() const { return *d_; }
@@ -1306,6 +1324,8 @@ void compile_batch_typed(const BatchItem* items, std::size_t count, uint32_t thr
     prog.precision = sizeof(R) == 8 ? MH_C128 : MH_C64;
     std::vector<Compiled> comp(count);
     prog.passes.assign(count, {});
+    constexpr uint32_t kNoPhase = UINT32_MAX;
+    std::vector<uint32_t> phase_at(count, kNoPhase);
     std::atomic<std::size_t> next{0};
     std::exception_ptr error;
     std::mutex error_mutex;
@@ -1331,6 +1351,15 @@ void compile_batch_typed(const BatchItem* items, std::size_t count, uint32_t thr
                     }
                     prog.passes[i].push_back(rec);
                 }
+                // A state to initialise: its first pass generates it from the
+                // phase, kept at the end of its matrix entries.
+                if (it.initialise && !prog.passes[i].empty() && !prog.passes[i][0].generic) {
+                    phase_at[i] = static_cast<uint32_t>(c.mats.size());
+                    V ph;
+                    ph.x = static_cast<R>(it.phase_re);
+                    ph.y = static_cast<R>(it.phase_im);
+                    c.mats.push_back(ph);
+                }
             }
         } catch (...) {
             std::lock_guard<std::mutex> lock(error_mutex);
@@ -1345,6 +1374,9 @@ void compile_batch_typed(const BatchItem* items, std::size_t count, uint32_t thr
         for (auto& th : pool) th.join();
     }
     if (error) std::rethrow_exception(error);
+    prog.needs_init.clear();
+    for (std::size_t i = 0; i < count; ++i)
+        if (items[i].initialise && phase_at[i] == kNoPhase) prog.needs_init.push_back(i);
 
     // Every state's tables, packed one after another.
     std::vector<std::size_t> bundle_base(count), op_base(count), term_base(count), mat_base(count);
@@ -1438,6 +1470,10 @@ void compile_batch_typed(const BatchItem* items, std::size_t count, uint32_t thr
                 a.op_off = static_cast<uint32_t>(op_base[i] + rec.op_begin);
                 a.term_off = static_cast<uint32_t>(term_base[i]);
                 a.mat_off = static_cast<uint32_t>(mat_base[i]);
+                if (p == 0 && phase_at[i] != kNoPhase) {
+                    a.generate = 1;
+                    a.phase_at = phase_at[i];
+                }
                 a.lay = rec.lay;
                 if (uses_pairs(comp[i].bundles.data() + rec.bundle_begin, rec.lay.num_bundles, Shape<R>::kRegBits))
                     l.pairs = true;
@@ -1470,6 +1506,19 @@ uint32_t issue_batch_typed(const BatchProgramData& prog, const BatchItem* items,
     const uint32_t nstreams =
         prog.has_generic ? 1u : std::max(1u, std::min(sched.kernel_streams, Streams::kKernelStreams));
     uint32_t launches = 0;
+    // States that no window pass can generate start from an initialisation
+    // kernel on their launch group's stream.
+    if (!prog.needs_init.empty()) {
+        std::vector<uint32_t> group_of(prog.passes.size(), 0);
+        for (uint32_t lg = 0; lg < prog.lgroups.size(); ++lg)
+            for (std::size_t i : prog.lgroups[lg]) group_of[i] = lg;
+        for (std::size_t i : prog.needs_init) {
+            const hipStream_t stream =
+                prog.has_generic ? streams().main : streams().kernel((first_stream + group_of[i]) % nstreams);
+            launch_init_on(items[i].state, items[i].n, prog.precision, items[i].phase_re, items[i].phase_im, stream);
+            ++launches;
+        }
+    }
     for (const BatchLaunch& l : prog.launches) {
         const hipStream_t stream = prog.has_generic ? streams().main : streams().kernel((first_stream + l.group) % nstreams);
         if (l.len > 0) {

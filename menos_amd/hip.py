@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmenos_hip.so")
 
 C128, C64 = 0, 1
-FUSE_NONE, FUSE_CONTAINMENT, FUSE_WIDTH, FUSE_PAIRS = 0, 1, 2, 3
+FUSE_NONE, FUSE_CONTAINMENT, FUSE_WIDTH, FUSE_PAIRS, FUSE_AUTO = 0, 1, 2, 3, 4
 MH_OK, MH_ERR_INVALID, MH_ERR_RESOURCE, MH_ERR_INTERNAL, MH_ERR_DEVICE, MH_ERR_ABANDONED = range(6)
 
 # int (*)(void* user_data, uint64_t boundaries_crossed): nonzero abandons.
@@ -86,20 +86,23 @@ class CResult(ctypes.Structure):
                 ("gates_absorbed", ctypes.c_uint64), ("actual_bytes", ctypes.c_uint64),
                 ("peak_device_bytes", ctypes.c_uint64), ("execute_seconds", ctypes.c_double),
                 ("gate_seconds", ctypes.c_double), ("sample_seconds", ctypes.c_double),
-                ("expectation_seconds", ctypes.c_double)]
+                ("expectation_seconds", ctypes.c_double), ("fusion_used", ctypes.c_int32),
+                ("fusion_width_used", ctypes.c_uint32)]
 
 
 class CCost(ctypes.Structure):
     _fields_ = [("feasible", ctypes.c_int32), ("infeasible_reason", ctypes.c_char * 256),
                 ("device_bytes", ctypes.c_uint64), ("host_bytes", ctypes.c_uint64), ("work_units", ctypes.c_uint64),
                 ("blocks", ctypes.c_uint32), ("passes", ctypes.c_uint32), ("launches", ctypes.c_uint64),
-                ("h2d_bytes", ctypes.c_uint64), ("d2h_bytes", ctypes.c_uint64), ("seconds", ctypes.c_double)]
+                ("h2d_bytes", ctypes.c_uint64), ("d2h_bytes", ctypes.c_uint64), ("seconds", ctypes.c_double),
+                ("fusion_used", ctypes.c_int32), ("fusion_width_used", ctypes.c_uint32)]
 
 
 class Calibration(ctypes.Structure):
     _fields_ = [("hbm_bytes_per_second", ctypes.c_double), ("pass_bytes_per_second", ctypes.c_double),
                 ("launch_seconds", ctypes.c_double), ("h2d_bytes_per_second", ctypes.c_double),
-                ("d2h_bytes_per_second", ctypes.c_double), ("fp64_ops_per_second", ctypes.c_double)]
+                ("d2h_bytes_per_second", ctypes.c_double), ("fp64_ops_per_second", ctypes.c_double),
+                ("mfma_flops_per_second", ctypes.c_double)]
 
 
 _lib = None
@@ -379,7 +382,8 @@ class HipDenseEngine:
                       gates_absorbed=res.gates_absorbed, actual_bytes=res.actual_bytes,
                       peak_device_bytes=res.peak_device_bytes, execute_seconds=res.execute_seconds,
                       gate_seconds=res.gate_seconds, sample_seconds=res.sample_seconds,
-                      expectation_seconds=res.expectation_seconds)
+                      expectation_seconds=res.expectation_seconds, fusion_used=res.fusion_used,
+                      fusion_width_used=res.fusion_width_used)
 
 
 class CBatchRequest(ctypes.Structure):

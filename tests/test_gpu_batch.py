@@ -4,7 +4,7 @@ to the single engine's bit for bit, complex64 within the stated error."""
 import numpy as np
 import pytest
 
-from circuits import diagonal_heavy_circuit, random_mixed_circuit
+from circuits import diagonal_heavy_circuit, random_mixed_circuit, random_unitary
 from menos_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -15,8 +15,7 @@ def _mixed_batch(count, seed):
     circuits, terms = [], []
     for i in range(count):
         n = int(rng.integers(10, 15))
-        # Blocks of at most two qubits: wider ones run on the generic kernel,
-        # whose GPU cases are covered (and held back) elsewhere.
+        # Blocks of at most two qubits (wider ones: test_batch_wide_and_empty_circuits).
         if i % 4 == 0:
             c = W.random_circuit(n, int(rng.integers(1, 5)), seed + i)
         elif i % 4 == 1:
@@ -90,3 +89,31 @@ def test_batch_refuses_small_states(hip):
     c = W.random_circuit(8, 2, 1)
     with pytest.raises(hip.HipError):
         hip.batch_execute([c], [[(0, 1)]])
+
+
+def test_batch_wide_and_empty_circuits(orc, hip):
+    """States whose first pass is not a window pass (a first block of three
+    or more qubits, on the per-group kernel, or no gate at all) start from an
+    initialisation kernel; the others generate their initial state in their
+    first window pass. Every state and term is checked, with global phases."""
+    circuits, terms = [], []
+    for i in range(10):
+        n = 10 + i % 4
+        if i % 3 == 0:
+            c = random_mixed_circuit(n, 50, 900 + i, max_unitary=4, max_arity=3)
+            u = random_unitary(3, np.random.default_rng(i))
+            c.ops.insert(0, (W.UNITARY, (0, 4, 2), tuple(np.stack([u.real, u.imag], -1).reshape(-1).tolist())))
+        elif i % 3 == 1:
+            c = W.Circuit(n, 0, 0.37 * i)
+        else:
+            c = W.random_circuit(n, 3, 900 + i)
+        c = _unitary_only(c)
+        c.global_phase = 0.21 * i
+        circuits.append(c)
+        terms.append(W.random_paulis(n, 6, 900 + i, max_weight=3))
+    for fusion in (0, 1):
+        r = hip.batch_execute(circuits, terms, fusion=fusion, statevectors=True)
+        for i, c in enumerate(circuits):
+            want, _, e = orc.execute(c, fuse_gates=bool(fusion), terms=terms[i])
+            assert r.statevectors[i].tobytes() == want.tobytes(), (fusion, i)
+            assert np.max(np.abs(r.expectation[i] - e)) <= 1e-12, (fusion, i)

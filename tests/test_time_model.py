@@ -12,7 +12,8 @@ from menos_amd import hip as H
 from menos_amd import workloads as W
 
 PROFILE = dict(hbm_bytes_per_second=6.0e12, pass_bytes_per_second=5.0e12, launch_seconds=1.0e-5,
-               h2d_bytes_per_second=5.0e10, d2h_bytes_per_second=4.0e10, fp64_ops_per_second=3.0e13)
+               h2d_bytes_per_second=5.0e10, d2h_bytes_per_second=4.0e10, fp64_ops_per_second=3.0e13,
+               mfma_flops_per_second=1.0e14)
 
 
 @pytest.fixture
@@ -74,3 +75,50 @@ def test_profile_rates_must_be_positive():
     bad = dict(PROFILE, launch_seconds=float("nan"))
     with pytest.raises(H.HipError):
         H.set_calibration(H.Calibration(**bad))
+
+
+def deep_circuit(n, first, k, layers, seed):
+    """`layers` layers of U3 on qubits first..first+k-1 and a CX brick among them:
+    all the work on k qubits of an n-qubit state."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    c = W.Circuit(n)
+    for layer in range(layers):
+        for q in range(first, first + k):
+            c.append(W.U, (q,), tuple(rng.uniform(0.0, 6.28, 3)))
+        for q in range(first + layer % 2, first + k - 1, 2):
+            c.append(W.CX, (q, q + 1))
+    return c
+
+
+def test_autotuner_keeps_window_passes_for_the_random_circuit(profile):
+    eng = H.HipDenseEngine()
+    for n in (24, 30, 33):
+        c = W.random_circuit(n, 16, 2025)
+        for prec in (H.C128, H.C64):
+            auto = eng.estimate(c, precision=prec, fusion=H.FUSE_AUTO)
+            plain = eng.estimate(c, precision=prec)
+            assert (auto.fusion_used, auto.fusion_width_used) == (H.FUSE_CONTAINMENT, 0), (n, prec)
+            assert auto.passes == plain.passes and auto.seconds == plain.seconds
+
+
+@pytest.mark.parametrize("k", [4, 5, 6])
+def test_autotuner_fuses_a_deep_narrow_circuit(profile, k):
+    eng = H.HipDenseEngine()
+    c = deep_circuit(24, 18, k, 40, k)
+    for prec in (H.C128, H.C64):
+        auto = eng.estimate(c, precision=prec, fusion=H.FUSE_AUTO)
+        plain = eng.estimate(c, precision=prec)
+        assert (auto.fusion_used, auto.fusion_width_used) == (H.FUSE_WIDTH, k), prec
+        assert auto.passes == 1 and auto.seconds < plain.seconds
+
+
+def test_autotuner_follows_the_profile(profile):
+    """With arithmetic a thousand times slower, one wide block costs more than
+    the window passes it replaces: the containment plan is kept."""
+    H.set_calibration(H.Calibration(**dict(PROFILE, fp64_ops_per_second=3.0e10, mfma_flops_per_second=1.0e11)))
+    eng = H.HipDenseEngine()
+    c = deep_circuit(24, 18, 6, 40, 6)
+    for prec in (H.C128, H.C64):
+        auto = eng.estimate(c, precision=prec, fusion=H.FUSE_AUTO)
+        assert auto.fusion_used == H.FUSE_CONTAINMENT, prec
