@@ -136,8 +136,9 @@ def batched(args, H, W) -> dict:
 def batched_graph(args, H, circuits, terms, fusion, per) -> dict:
     """The variational loop on the resident, graph-captured batch
     (mh_batch_graph_*): the same circuits with new angles every iteration.
-    An iteration is update (block matrices and pass programs rebuilt on host
-    threads, the matrix table staged) plus run (one graph launch: matrix upload,
+    An iteration is set_params (one array of every circuit's angles; block
+    matrices and pass programs rebuilt on host threads, the matrix table
+    staged) plus run (one graph launch: matrix upload,
     initialisation, every pass, the term reductions, the terms back). Rates are
     over `args.graph_iterations` iterations after one untimed one; the last
     iteration's terms are checked against mh_batch_execute on the same angles
@@ -152,10 +153,12 @@ def batched_graph(args, H, circuits, terms, fusion, per) -> dict:
             g.run()
             upd_s = run_s = 0.0
             lat = []
+            sizes = [c.num_angles for c in circuits]
+            assert g.param_count() == sum(sizes)
             for it in range(args.graph_iterations + 1):
-                new = [c.with_angles(rng.uniform(-np.pi, np.pi, c.num_angles)) for c in circuits]
+                params = rng.uniform(-np.pi, np.pi, sum(sizes))
                 u0 = time.perf_counter()
-                g.update(new)
+                g.set_params(params)
                 u1 = time.perf_counter()
                 e, res = g.run()
                 u2 = time.perf_counter()
@@ -164,6 +167,8 @@ def batched_graph(args, H, circuits, terms, fusion, per) -> dict:
                 upd_s += u1 - u0
                 run_s += u2 - u1
                 lat.append((u2 - u0) * 1e3)
+            cuts = np.cumsum([0] + sizes)
+            new = [c.with_angles(params[cuts[i]:cuts[i + 1]]) for i, c in enumerate(circuits)]
             check = H.batch_execute(new, terms, precision=prec, fusion=fusion[prec], **sched).expectation
             same = all(a.tobytes() == b.tobytes() for a, b in zip(e, check))
             last[name] = e

@@ -420,6 +420,11 @@ int mh_batch_execute(const mh_batch_request* request, mh_batch_result* result);
 typedef struct mh_batch_graph mh_batch_graph;
 int mh_batch_graph_create(const mh_batch_request* request, mh_batch_graph** out);
 int mh_batch_graph_update(mh_batch_graph* graph, const mh_circuit* circuits, const double* global_phases);
+/* The same with the parameters alone: every circuit's parameters, in circuit
+ * order, concatenated (mh_batch_graph_param_count of them; each circuit's
+ * params laid out as in its mh_circuit), and the global phases (null: 0). */
+int mh_batch_graph_set_params(mh_batch_graph* graph, const double* params, const double* global_phases);
+int mh_batch_graph_param_count(const mh_batch_graph* graph, uint64_t* out);
 int mh_batch_graph_run(mh_batch_graph* graph, mh_batch_result* result);
 int mh_batch_graph_destroy(mh_batch_graph* graph);
 

@@ -9,10 +9,12 @@
 //   1. every circuit is validated and planned (host threads);
 //   2. circuits are taken in order into rounds whose states fit round_bytes;
 //      a round's states share one pooled device arena;
-//   3. per round: one clear of the arena plus one kernel that writes each
-//      state's first amplitude (the global phase), then pass p of every
-//      circuit in one launch (run_window_passes_batch), then every circuit's
-//      expectation terms in two launches (expectation_batch), one copy back.
+//   3. per round, in groups of circuits (the host plans a group while the
+//      device runs the one before): pass p of every circuit of the group in
+//      one launch (BatchPassRunner; the first pass writes each state, global
+//      phase included, so the arena is never cleared), then the group's
+//      expectation terms in two launches behind its passes; one copy back
+//      per group at the end.
 //
 // The passes and reductions are the single-circuit engine's kernels, so
 // results equal mh_engine_execute's bit for bit.
@@ -85,7 +87,6 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
     unsigned char* const arena = static_cast<unsigned char*>(arena_in ? arena_in : own.get());
     const std::size_t count = items.size();
     std::vector<BatchItem> passes(count);
-    std::vector<ExpectationItem> exp_items;
     uint64_t at = 0;
     for (std::size_t k = 0; k < count; ++k) {
         const RoundItem& it = items[k];
@@ -96,16 +97,6 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
         passes[k].initialise = true;
         passes[k].phase_re = ph.real();
         passes[k].phase_im = ph.imag();
-        if (it.num_terms > 0) {
-            ExpectationItem e;
-            e.state = passes[k].state;
-            e.n = it.n;
-            e.terms = it.num_terms;
-            e.x = it.x;
-            e.z = it.z;
-            e.out_index = it.term_out;
-            exp_items.push_back(e);
-        }
         at += round_state_bytes(it.n, precision);
     }
     // On an error past this point, wait for the queued work before the
@@ -125,15 +116,30 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
     const auto p0 = clock::now();
     BatchStats bs;
     double prepare_seconds = 0;
+    double reduce_seconds = 0;
+    uint32_t launches = 0;
+    // Each group's reduction, with its workspace; read back after the passes.
+    struct GroupReduction {
+        std::unique_ptr<ExpectationProgram> program;
+        DeviceBuffer workspace;
+        hipEvent_t uploaded = nullptr;
+        ~GroupReduction() {
+            if (uploaded) (void)hipEventDestroy(uploaded);
+        }
+    };
+    std::vector<std::unique_ptr<GroupReduction>> reductions;
     {
         // Groups of items: the host plans and compiles group g + 1 while the
         // device runs group g. Group sizes grow (1, 1, 2, 4, 8 sixteenths) so
         // that only a sixteenth of the host work is exposed before the device
         // starts, and every later group's host work is shorter than the device
-        // work queued before it.
+        // work queued before it. A group's terms are reduced on the stream its
+        // passes end on, behind them, so the reductions of early groups overlap
+        // the passes of later ones.
         BatchPassRunner runner(precision, threads, schedule);
         static constexpr std::size_t kEnds[] = {1, 2, 4, 8, 16};
         const std::size_t groups = count >= 64 ? 5 : 1;
+        reductions.reserve(groups);
         for (std::size_t g = 0; g < groups; ++g) {
             const std::size_t lo = groups == 1 ? 0 : count * (g ? kEnds[g - 1] : 0) / 16;
             const std::size_t hi = groups == 1 ? count : count * kEnds[g] / 16;
@@ -146,17 +152,40 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
                 if (items[k].blocks == nullptr) fail(MH_ERR_INTERNAL, "a round item without blocks");
                 passes[k].blocks = items[k].blocks;
             }
-            runner.add(passes.data() + lo, hi - lo, &bs);
+            const hipStream_t tail = runner.add(passes.data() + lo, hi - lo, &bs);
+            const auto r0 = clock::now();
+            std::vector<ExpectationItem> group_terms;
+            for (std::size_t k = lo; k < hi; ++k) {
+                const RoundItem& it = items[k];
+                if (it.num_terms == 0) continue;
+                ExpectationItem e;
+                e.state = passes[k].state;
+                e.n = it.n;
+                e.terms = it.num_terms;
+                e.x = it.x;
+                e.z = it.z;
+                e.out_index = it.term_out;
+                group_terms.push_back(e);
+            }
+            if (!group_terms.empty()) {
+                reductions.push_back(std::make_unique<GroupReduction>());
+                GroupReduction& red = *reductions.back();
+                red.program = std::make_unique<ExpectationProgram>(precision, group_terms);
+                red.workspace = DeviceBuffer(red.program->workspace_bytes());
+                red.program->upload_on(red.workspace.get(), streams().copy);
+                MH_HIP(hipEventCreateWithFlags(&red.uploaded, hipEventDisableTiming));
+                MH_HIP(hipEventRecord(red.uploaded, streams().copy));
+                MH_HIP(hipStreamWaitEvent(tail, red.uploaded, 0));
+                red.program->launch_on(red.workspace.get(), tail);
+                launches += 2;
+            }
+            reduce_seconds += std::chrono::duration<double>(clock::now() - r0).count();
         }
         runner.finish();
     }
     const auto p1 = clock::now();
-    uint32_t launches = bs.launches;
-    if (!exp_items.empty()) {
-        expectation_batch(precision, exp_items, terms_out);
-        synchronize();
-        launches += 2;
-    }
+    launches += bs.launches;
+    for (const auto& red : reductions) red->program->read(red->workspace.get(), terms_out);
     const auto p2 = clock::now();
     for (std::size_t k = 0; k < count; ++k) {
         const RoundItem& it = items[k];
@@ -174,8 +203,9 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
         stats->launches += launches;
         stats->compile_seconds += bs.compile_seconds;
         stats->prepare_seconds += prepare_seconds;
-        stats->pass_seconds += std::chrono::duration<double>(p1 - p0).count() - bs.compile_seconds - prepare_seconds;
-        stats->expectation_seconds += std::chrono::duration<double>(p2 - p1).count();
+        stats->pass_seconds +=
+            std::chrono::duration<double>(p1 - p0).count() - bs.compile_seconds - prepare_seconds - reduce_seconds;
+        stats->expectation_seconds += std::chrono::duration<double>(p2 - p1).count() + reduce_seconds;
         stats->sample_seconds += std::chrono::duration<double>(p3 - p2).count();
     }
 }

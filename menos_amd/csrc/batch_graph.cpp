@@ -216,6 +216,14 @@ void capture(mh_batch_graph& g) {
 }  // namespace
 }  // namespace mh
 
+namespace mh {
+namespace {
+// New parameters (circuits of the captured structure) into the staged matrix
+// table; refuses a change of any pass program.
+void update_blocks(mh_batch_graph& g, const mh_circuit* circuits, const double* global_phases);
+}  // namespace
+}  // namespace mh
+
 using namespace mh;
 
 extern "C" {
@@ -329,6 +337,43 @@ int mh_batch_graph_update(mh_batch_graph* g, const mh_circuit* circuits, const d
             if (!same_structure(circuits[i], g->structure[i].view))
                 fail(MH_ERR_INVALID, "circuit " + std::to_string(i) +
                                          " differs in structure from the captured one (only parameters may change)");
+        update_blocks(*g, circuits, global_phases);
+    });
+}
+
+int mh_batch_graph_set_params(mh_batch_graph* g, const double* params, const double* global_phases) {
+    return guarded([&] {
+        if (g == nullptr) fail(MH_ERR_INVALID, "null graph");
+        std::vector<mh_circuit> views(g->count);
+        uint64_t at = 0;
+        for (uint32_t i = 0; i < g->count; ++i) {
+            views[i] = g->structure[i].view;
+            const uint64_t np = views[i].param_offsets[views[i].num_ops];
+            if (np > 0 && params == nullptr) fail(MH_ERR_INVALID, "null parameters");
+            views[i].params = np > 0 ? params + at : nullptr;
+            at += np;
+        }
+        update_blocks(*g, views.data(), global_phases);
+    });
+}
+
+int mh_batch_graph_param_count(const mh_batch_graph* g, uint64_t* out) {
+    return guarded([&] {
+        if (g == nullptr || out == nullptr) fail(MH_ERR_INVALID, "null graph or output");
+        uint64_t total = 0;
+        for (const Structure& st : g->structure) total += st.view.param_offsets[st.view.num_ops];
+        *out = total;
+    });
+}
+
+}  // extern "C"
+
+namespace mh {
+namespace {
+
+void update_blocks(mh_batch_graph& gr, const mh_circuit* circuits, const double* global_phases) {
+    mh_batch_graph* g = &gr;
+    {
         std::vector<std::vector<Block>> fresh(g->count);
         const auto t0 = clock_type::now();
         parallel_for(g->count, g->threads, [&](std::size_t i) {
@@ -351,8 +396,13 @@ int mh_batch_graph_update(mh_batch_graph* g, const mh_circuit* circuits, const d
         g->items = std::move(items);
         for (uint32_t i = 0; i < g->count; ++i) g->items[i].blocks = &g->blocks[i];
         g->program = std::move(prog);
-    });
+    }
 }
+
+}  // namespace
+}  // namespace mh
+
+extern "C" {
 
 int mh_batch_graph_run(mh_batch_graph* g, mh_batch_result* res) {
     return guarded([&] {

@@ -590,30 +590,34 @@ ExpPlan plan_expectation(const std::vector<ExpectationItem>& items) {
     return P;
 }
 
-// Copies a plan's descriptors into its workspace (main stream, not waited for).
-void upload_expectation(const ExpPlan& P, void* ws) {
+// Copies a plan's descriptors into its workspace on `stream` (not waited for;
+// the plan's host vectors must outlive the copy).
+void upload_expectation(const ExpPlan& P, void* ws, hipStream_t stream) {
     char* w = static_cast<char*>(ws);
-    upload(w + P.at_slots, P.slots.data(), P.slots.size() * sizeof(ExpSlot));
+    auto put = [&](std::size_t at, const void* src, std::size_t bytes) {
+        if (bytes) MH_HIP(hipMemcpyAsync(w + at, src, bytes, hipMemcpyHostToDevice, stream));
+    };
+    put(P.at_slots, P.slots.data(), P.slots.size() * sizeof(ExpSlot));
     if (!P.tjobs.empty()) {
-        upload(w + P.at_tjobs, P.tjobs.data(), P.tjobs.size() * sizeof(TileJob));
-        upload(w + P.at_tterms, P.tterms.data(), P.tterms.size() * sizeof(TileTerm));
-        upload(w + P.at_tbegin, P.tblock_begin.data(), P.tblock_begin.size() * sizeof(uint32_t));
+        put(P.at_tjobs, P.tjobs.data(), P.tjobs.size() * sizeof(TileJob));
+        put(P.at_tterms, P.tterms.data(), P.tterms.size() * sizeof(TileTerm));
+        put(P.at_tbegin, P.tblock_begin.data(), P.tblock_begin.size() * sizeof(uint32_t));
     }
     if (!P.jobs.empty()) {
-        upload(w + P.at_jobs, P.jobs.data(), P.jobs.size() * sizeof(ExpJob));
-        upload(w + P.at_begin, P.block_begin.data(), P.block_begin.size() * sizeof(uint32_t));
+        put(P.at_jobs, P.jobs.data(), P.jobs.size() * sizeof(ExpJob));
+        put(P.at_begin, P.block_begin.data(), P.block_begin.size() * sizeof(uint32_t));
     }
 }
 
 // The reduction kernels of a plan whose descriptors are in the workspace
-// (main stream; no copies).
+// (on `stream`; no copies).
 template <typename V>
-void launch_expectation(const ExpPlan& P, void* ws) {
+void launch_expectation(const ExpPlan& P, void* ws, hipStream_t stream) {
     char* w = static_cast<char*>(ws);
     double* partials = reinterpret_cast<double*>(w + P.at_partials);
     double* dout = reinterpret_cast<double*>(w + P.at_out);
     if (!P.tjobs.empty()) {
-        hipLaunchKernelGGL(expectation_tile_kernel<V>, dim3(P.tile_blocks), dim3(kThreads), 0, streams().main,
+        hipLaunchKernelGGL(expectation_tile_kernel<V>, dim3(P.tile_blocks), dim3(kThreads), 0, stream,
                            reinterpret_cast<const TileJob*>(w + P.at_tjobs),
                            reinterpret_cast<const uint32_t*>(w + P.at_tbegin), static_cast<uint32_t>(P.tjobs.size()),
                            reinterpret_cast<const TileTerm*>(w + P.at_tterms), reinterpret_cast<const V*>(P.base_addr),
@@ -621,14 +625,14 @@ void launch_expectation(const ExpPlan& P, void* ws) {
         MH_HIP(hipGetLastError());
     }
     if (!P.jobs.empty()) {
-        hipLaunchKernelGGL(expectation_batch_kernel<V>, dim3(P.total_blocks), dim3(kThreads), 0, streams().main,
+        hipLaunchKernelGGL(expectation_batch_kernel<V>, dim3(P.total_blocks), dim3(kThreads), 0, stream,
                            reinterpret_cast<const ExpJob*>(w + P.at_jobs),
                            reinterpret_cast<const uint32_t*>(w + P.at_begin), static_cast<uint32_t>(P.jobs.size()),
                            reinterpret_cast<const V*>(P.base_addr), partials);
         MH_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(expectation_finish_batch_kernel, dim3(static_cast<uint32_t>(P.slots.size())), dim3(kThreads), 0,
-                       streams().main, reinterpret_cast<const ExpSlot*>(w + P.at_slots),
+                       stream, reinterpret_cast<const ExpSlot*>(w + P.at_slots),
                        static_cast<const double*>(partials), dout);
     MH_HIP(hipGetLastError());
 }
@@ -640,8 +644,8 @@ void launch_expectation(const ExpPlan& P, void* ws) {
 template <typename V>
 void run_expectation(const ExpPlan& P, void* ws, double* out) {
     if (P.slots.empty()) return;
-    upload_expectation(P, ws);
-    launch_expectation<V>(P, ws);
+    upload_expectation(P, ws, streams().main);
+    launch_expectation<V>(P, ws, streams().main);
     const double* dout = reinterpret_cast<const double*>(static_cast<const char*>(ws) + P.at_out);
     std::vector<double> host(uint64_t(P.out_terms) * 2);
     download(host.data(), dout, host.size() * sizeof(double));
@@ -681,17 +685,41 @@ uint32_t ExpectationProgram::out_terms() const { return impl_->plan.out_terms; }
 
 void ExpectationProgram::upload(void* workspace) const {
     if (impl_->plan.slots.empty()) return;
-    upload_expectation(impl_->plan, workspace);
+    upload_expectation(impl_->plan, workspace, streams().main);
     synchronize();
 }
 
-void ExpectationProgram::launch(void* workspace) const {
+void ExpectationProgram::upload_on(void* workspace, hipStream_t stream) const {
+    if (impl_->plan.slots.empty()) return;
+    upload_expectation(impl_->plan, workspace, stream);
+}
+
+void ExpectationProgram::launch(void* workspace) const { launch_on(workspace, streams().main); }
+
+void ExpectationProgram::launch_on(void* workspace, hipStream_t stream) const {
     if (impl_->plan.slots.empty()) return;
     if (impl_->precision == MH_C128)
-        launch_expectation<double2>(impl_->plan, workspace);
+        launch_expectation<double2>(impl_->plan, workspace, stream);
     else
-        launch_expectation<float2>(impl_->plan, workspace);
+        launch_expectation<float2>(impl_->plan, workspace, stream);
 }
+
+void ExpectationProgram::read(const void* workspace, double* out) const {
+    const ExpPlan& P = impl_->plan;
+    if (P.slots.empty()) return;
+    // Only the slots from the lowest one this program writes.
+    uint64_t lo = P.out_terms;
+    for (const ExpSlot& sl : P.slots) lo = std::min<uint64_t>(lo, sl.out);
+    std::vector<double> host((P.out_terms - lo) * 2);
+    download(host.data(), static_cast<const char*>(workspace) + P.at_out + lo * 2 * sizeof(double),
+             host.size() * sizeof(double));
+    for (const ExpSlot& sl : P.slots) {
+        out[2 * uint64_t(sl.out)] = host[2 * (sl.out - lo)];
+        out[2 * uint64_t(sl.out) + 1] = host[2 * (sl.out - lo) + 1];
+    }
+}
+
+bool ExpectationProgram::empty() const { return impl_->plan.slots.empty(); }
 
 void expectation_batch(int32_t precision, const std::vector<ExpectationItem>& items, double* out) {
     amp_bytes(precision);

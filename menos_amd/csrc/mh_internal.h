@@ -193,9 +193,15 @@ public:
     // Copies the descriptors into the workspace on the calling thread's main
     // stream and waits for the copy.
     void upload(void* workspace) const;
+    // The same copy on `stream`, not waited for (the program outlives it).
+    void upload_on(void* workspace, hipStream_t stream) const;
     // Launches the reduction kernels on the main stream (no copies, no
     // allocation, no waits: capturable).
     void launch(void* workspace) const;
+    void launch_on(void* workspace, hipStream_t stream) const;
+    // Copies the terms back (main stream, waited for) into out[2 * term].
+    void read(const void* workspace, double* out) const;
+    bool empty() const;
 
 private:
     struct Impl;

@@ -1240,6 +1240,13 @@ struct BatchKeep {
     std::vector<DeviceBuffer> device;
     std::vector<std::shared_ptr<void>> host;
     hipEvent_t copied = nullptr;
+    std::vector<hipEvent_t> joins;  // launch groups' ends, waited for on the tail stream
+    void release() {
+        if (copied) (void)hipEventDestroy(copied);
+        copied = nullptr;
+        for (hipEvent_t e : joins) (void)hipEventDestroy(e);
+        joins.clear();
+    }
 };
 
 
@@ -1565,9 +1572,12 @@ void keep_table(BatchKeep& keep, const std::vector<unsigned char>& src, hipStrea
 // Many states, each with its own block list, compiled into one program whose
 // tables go up on the copy stream; its launches then wait for them on the
 // kernel streams (see issue_batch_program). Returns without waiting; `keep`
-// holds what the launches read.
-void launch_window_passes_batch(int32_t precision, const BatchItem* items, std::size_t count, uint32_t threads,
-                                BatchStats* stats, BatchKeep& keep, const BatchSchedule& sched, uint32_t first_stream) {
+// holds what the launches read. Returns the stream on which work queued next
+// runs after every pass of these items (the main stream when a pass is
+// generic, else the first launch group's stream, made to wait for the others).
+hipStream_t launch_window_passes_batch(int32_t precision, const BatchItem* items, std::size_t count, uint32_t threads,
+                                       BatchStats* stats, BatchKeep& keep, const BatchSchedule& sched,
+                                       uint32_t first_stream) {
     context();
     const auto c0 = std::chrono::steady_clock::now();
     const BatchProgram prog = compile_batch_program(precision, items, count, threads, sched);
@@ -1598,6 +1608,18 @@ void launch_window_passes_batch(int32_t precision, const BatchItem* items, std::
         stats->launches += launches;
         stats->max_passes = std::max(stats->max_passes, prog.data().max_passes);
     }
+    if (prog.has_generic()) return streams().main;
+    const uint32_t nstreams = std::max(1u, std::min(sched.kernel_streams, Streams::kKernelStreams));
+    const hipStream_t tail = streams().kernel(first_stream % nstreams);
+    const std::size_t used = std::min<std::size_t>(prog.data().lgroups.size(), nstreams);
+    for (std::size_t g = 1; g < used; ++g) {
+        hipEvent_t e = nullptr;
+        MH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        keep.joins.push_back(e);
+        MH_HIP(hipEventRecord(e, streams().kernel((first_stream + g) % nstreams)));
+        MH_HIP(hipStreamWaitEvent(tail, e, 0));
+    }
+    return tail;
 }
 
 }  // namespace
@@ -1703,13 +1725,12 @@ BatchPassRunner::~BatchPassRunner() {
         (void)hipStreamSynchronize(streams().copy);
         for (uint32_t k = 0; k < Streams::kKernelStreams; ++k) (void)hipStreamSynchronize(streams().kernel(k));
     }
-    for (BatchKeep& k : impl_->keeps)
-        if (k.copied) (void)hipEventDestroy(k.copied);
+    for (BatchKeep& k : impl_->keeps) k.release();
     if (impl_->start) (void)hipEventDestroy(impl_->start);
 }
 
-void BatchPassRunner::add(const BatchItem* items, std::size_t count, BatchStats* stats) {
-    if (count == 0) return;
+hipStream_t BatchPassRunner::add(const BatchItem* items, std::size_t count, BatchStats* stats) {
+    if (count == 0) return streams().main;
     const uint32_t width = window_width(impl_->precision);
     for (std::size_t i = 0; i < count; ++i)
         if (items[i].n < width) fail(MH_ERR_INVALID, "a batched state has fewer qubits than one window");
@@ -1718,7 +1739,7 @@ void BatchPassRunner::add(const BatchItem* items, std::size_t count, BatchStats*
     // the end of one call overlap the next call's.
     context();
     const uint32_t first_stream = impl_->added++;
-    launch_window_passes_batch(impl_->precision, items, count, impl_->threads, stats, impl_->keeps.back(),
+    return launch_window_passes_batch(impl_->precision, items, count, impl_->threads, stats, impl_->keeps.back(),
                                impl_->schedule, first_stream);
 }
 
@@ -1726,8 +1747,7 @@ void BatchPassRunner::finish() {
     MH_HIP(hipStreamSynchronize(streams().copy));
     for (uint32_t k = 1; k < Streams::kKernelStreams; ++k) MH_HIP(hipStreamSynchronize(streams().kernel(k)));
     synchronize();
-    for (BatchKeep& k : impl_->keeps)
-        if (k.copied) (void)hipEventDestroy(k.copied);
+    for (BatchKeep& k : impl_->keeps) k.release();
     impl_->keeps.clear();
 }
 

@@ -33,7 +33,7 @@ EXPORTS = (
     "mh_server_submit", "mh_server_wait", "mh_server_replay", "mh_server_stats_get", "mh_server_destroy",
     "mh_derive_seeds", "mh_set_calibration", "mh_get_calibration", "mh_expectation_noalloc",
     "mh_plan_passes", "mh_gather", "mh_batch_graph_create", "mh_batch_graph_update", "mh_batch_graph_run",
-    "mh_batch_graph_destroy",
+    "mh_batch_graph_destroy", "mh_batch_graph_set_params", "mh_batch_graph_param_count",
 )
 JOB_ESTIMATOR, JOB_SAMPLER = 0, 1
 
@@ -468,6 +468,18 @@ class BatchGraph:
         arr = (CCircuit * max(len(keeps), 1))(*[k.c for k in keeps])
         phases = np.array([c.global_phase for c in circuits] or [0.0], np.float64)
         check(lib().mh_batch_graph_update(self.ptr, arr, _ptr(phases, _f64p)))
+
+    def param_count(self) -> int:
+        n = ctypes.c_uint64()
+        check(lib().mh_batch_graph_param_count(self.ptr, ctypes.byref(n)))
+        return n.value
+
+    def set_params(self, params, global_phases=None):
+        """New parameters for every circuit, concatenated in circuit order."""
+        p = np.ascontiguousarray(params, np.float64)
+        ph = None if global_phases is None else np.ascontiguousarray(global_phases, np.float64)
+        check(lib().mh_batch_graph_set_params(self.ptr, _ptr(p, _f64p) if p.size else None,
+                                              _ptr(ph, _f64p) if ph is not None else None))
 
     def run(self):
         """Replays the graph: (one complex array of terms per circuit, CBatchResult)."""

@@ -53,6 +53,15 @@ def test_parameter_updates(hip):
             assert _same(got, want), step
             assert not _same(got, first)
             assert res.compile_seconds > 0
+        # The same through one parameter array.
+        sizes = [c.num_angles for c in circuits]
+        assert g.param_count() == sum(sizes)
+        params = rng.uniform(-np.pi, np.pi, sum(sizes))
+        g.set_params(params)
+        got, _ = g.run()
+        cuts = np.cumsum([0] + sizes)
+        updated = [c.with_angles(params[cuts[i]:cuts[i + 1]]) for i, c in enumerate(circuits)]
+        assert _same(got, hip.batch_execute(updated, terms, precision=1, fusion=3).expectation)
 
 
 def test_refusals_keep_the_graph(hip):
