@@ -115,6 +115,8 @@ def batched(args, H, W) -> dict:
     out["max_abs_err_c64_vs_c128"] = float(max(np.max(np.abs(a - b)) if a.size else 0.0
                                                for a, b in zip(res["c64"].expectation, res["c128"].expectation)))
     k = min(args.per_circuit_sample, len(circuits))
+    if k == 0:  # a profile of the batch alone
+        return out
     eng = H.HipDenseEngine()
     eng.execute(circuits[0], precision=H.C64, fusion=fusion[H.C64], terms=terms[0], statevector=False, targets=[],
                 width=0)
@@ -129,7 +131,8 @@ def batched(args, H, W) -> dict:
     out["speedup_c64"] = round(out["c64"]["circuits_per_s"] / per["c64"], 2)
     out["speedup_c128"] = round(out["c128"]["circuits_per_s"] / per["c128"], 2)
     log(f"per-circuit path on {k} circuits: {out['per_circuit_path']}; speedup c64 {out['speedup_c64']}x")
-    out["graph"] = batched_graph(args, H, circuits, terms, fusion, per)
+    if args.graph_iterations > 0:
+        out["graph"] = batched_graph(args, H, circuits, terms, fusion, per)
     return out
 
 

@@ -14,7 +14,7 @@
 // psi[m'], the pair contributes p * s(m') + conj(p) * s(m), s(k) =
 // (-1)^popcount(z & k); conj(p) is bitwise conj(psi[m']) * psi[m], so each
 // product is the reference's, read once. Each thread sums a fixed set of items
-// in order, then the workgroup and the partials are added in a fixed tree order:
+// in order, then the workgroup and the partials are added in a fixed order:
 // the value is reproducible run to run, the same for one state alone and in a
 // batch (both take this path), and differs from the
 // reference's sequential index-order sum by rounding only (the reference's
@@ -190,12 +190,15 @@ struct TileJob {
     uint8_t tpos[16];        // tile bit i -> qubit, ascending
 };
 
+// What a tile term sums: one real number per term (tile_term).
+enum TileKind : uint32_t { kDiagonalTerm = 0, kRealTerm = 1, kImagTerm = 2 };
+
 struct TileTerm {
     uint64_t x, z;
     uint32_t xl;             // x on the tile bits
     uint32_t hl;             // msb of xl (x != 0)
     uint32_t zl;             // z on the tile bits
-    uint32_t pad;
+    uint32_t kind;           // TileKind
     uint64_t partial_off;    // partials[(block) * 2 + {0, 1}] from here
 };
 
@@ -211,10 +214,16 @@ __device__ __forceinline__ double flip(double d, bool neg) {
 // over that split: the per-step parts are uniform, so each entry costs two
 // index XORs and one sign XOR. s(m ^ x) = s(m) ^ popcount(z & x). kFull: the
 // tile has 2^kTileBits entries, so no entry is out of range.
+//
+// Each term needs one real sum. x = 0 (kDiagonalTerm): s(m) |a_m|^2. x != 0:
+// the pair's s(m') p + s(m) conj(p), with s(m') = s(m) (-1)^sx and sx =
+// popcount(z & x) mod 2, is 2 s(m) Re p for sx = 0 (kRealTerm) and
+// 2i s(m') Im p for sx = 1 (kImagTerm); the factor 2 is applied to the
+// workgroup's sum (exact), and the other part is zero.
 template <bool kFull, typename V>
 __device__ __forceinline__ void tile_term(const V* tile, const TileTerm& tt, uint32_t pz, uint32_t u, uint32_t size,
-                                          double& accr, double& acci) {
-    if (tt.x == 0) {
+                                          double& acc) {
+    if (tt.kind == kDiagonalTerm) {
         const uint32_t su = (static_cast<uint32_t>(__popc(tt.zl & u)) ^ pz) & 1u;
 #pragma unroll
         for (uint32_t step = 0; step < (1u << kTileBits); step += kThreads) {
@@ -222,18 +231,18 @@ __device__ __forceinline__ void tile_term(const V* tile, const TileTerm& tt, uin
             if (!kFull && j >= size) continue;
             const double ar = static_cast<double>(tile[j].x), ai = static_cast<double>(tile[j].y);
             const double pr = ar * ar + ai * ai;
-            const double pi = ar * ai - ai * ar;
             const bool neg = ((su ^ static_cast<uint32_t>(__popc(tt.zl & step))) & 1u) != 0;
-            accr = accr + flip(pr, neg);
-            acci = acci + flip(pi, neg);
+            acc = acc + flip(pr, neg);
         }
-    } else {
-        // Pair r -> m: r's bits with a zero inserted at bit hl (the top bit of xl).
-        const uint32_t lowm = (1u << tt.hl) - 1u;
-        const uint32_t ju = ((u & ~lowm) << 1) | (u & lowm);
-        const uint32_t jpu = ju ^ tt.xl;
-        const uint32_t su = (static_cast<uint32_t>(__popc(tt.zl & ju)) ^ pz) & 1u;
-        const uint32_t sx = static_cast<uint32_t>(__popc(tt.zl & tt.xl)) & 1u;
+        return;
+    }
+    // Pair r -> m: r's bits with a zero inserted at bit hl (the top bit of xl).
+    const uint32_t lowm = (1u << tt.hl) - 1u;
+    const uint32_t ju = ((u & ~lowm) << 1) | (u & lowm);
+    const uint32_t jpu = ju ^ tt.xl;
+    // The sign taken: s(m) for a real term, s(m') = s(m) ^ 1 for an imaginary one.
+    const uint32_t su = (static_cast<uint32_t>(__popc(tt.zl & ju)) ^ pz ^ (tt.kind == kImagTerm ? 1u : 0u)) & 1u;
+    if (tt.kind == kRealTerm) {
 #pragma unroll
         for (uint32_t step = 0; step < (1u << kTileBits) / 2; step += kThreads) {
             if (!kFull && (step | u) >= size / 2) continue;
@@ -241,26 +250,30 @@ __device__ __forceinline__ void tile_term(const V* tile, const TileTerm& tt, uin
             const uint32_t j = ju | js, jp = jpu ^ js;
             const double ar = static_cast<double>(tile[j].x), ai = static_cast<double>(tile[j].y);
             const double br = static_cast<double>(tile[jp].x), bi = static_cast<double>(tile[jp].y);
-            // p = conj(a) * b, with a at m (bit h of m clear) and b at m ^ x
-            const double pr = ar * br + ai * bi;
-            const double pi = ar * bi - ai * br;
-            const uint32_t sm = (su ^ static_cast<uint32_t>(__popc(tt.zl & js))) & 1u;
-            const bool neg_m = sm != 0;           // s(m) for conj(p)
-            const bool neg_mp = (sm ^ sx) != 0;   // s(m') for p
-            accr = accr + flip(pr, neg_mp);
-            acci = acci + flip(pi, neg_mp);
-            accr = accr + flip(pr, neg_m);
-            acci = acci + flip(pi, !neg_m);
+            const double pr = ar * br + ai * bi;  // Re conj(a) b
+            const bool neg = ((su ^ static_cast<uint32_t>(__popc(tt.zl & js))) & 1u) != 0;
+            acc = acc + flip(pr, neg);
+        }
+    } else {
+#pragma unroll
+        for (uint32_t step = 0; step < (1u << kTileBits) / 2; step += kThreads) {
+            if (!kFull && (step | u) >= size / 2) continue;
+            const uint32_t js = ((step & ~lowm) << 1) | (step & lowm);
+            const uint32_t j = ju | js, jp = jpu ^ js;
+            const double ar = static_cast<double>(tile[j].x), ai = static_cast<double>(tile[j].y);
+            const double br = static_cast<double>(tile[jp].x), bi = static_cast<double>(tile[jp].y);
+            const double pi = ar * bi - ai * br;  // Im conj(a) b
+            const bool neg = ((su ^ static_cast<uint32_t>(__popc(tt.zl & js))) & 1u) != 0;
+            acc = acc + flip(pi, neg);
         }
     }
 }
 
-// Five waves per SIMD: the per-thread sums (16 terms, real and imaginary, in
-// double) would otherwise leave the kernel at 161 VGPRs (three waves), and the kernel waits on LDS and
-// memory latency. complex128 reaches four (its 32 KB tile caps the CU at four
-// workgroups) but still gets the smaller register budget, without spills,
-// which the four-wave bound does not give it; the compiler's note that five
-// is out of reach there is silenced.
+// Five waves per SIMD: the kernel waits on LDS and memory latency, and the
+// sixteen per-term sums in double fit beside the tile indices without spills.
+// complex128 reaches four (its 32 KB tile caps the CU at four workgroups) but
+// keeps the same register budget; the compiler's note that five is out of
+// reach there is silenced.
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Wpass-failed"
 template <typename V>
@@ -269,7 +282,7 @@ __global__ void __launch_bounds__(kThreads, 5)
                             uint32_t count, const TileTerm* __restrict__ terms, const V* base, double* partials) {
     __shared__ V tile[1u << kTileBits];
     __shared__ uint64_t dep_lo[64], dep_hi[32];
-    __shared__ double red[2][kThreads];
+    __shared__ double red[kThreads / 64][kTermsPerLaunch];
     const uint32_t b = blockIdx.x;
     uint32_t lo = 0, hi = count;
     while (hi - lo > 1) {
@@ -296,9 +309,9 @@ __global__ void __launch_bounds__(kThreads, 5)
     }
     const V* s = base + job.state_off;
     const uint32_t nt = job.term_count;
-    double accr[kTermsPerLaunch], acci[kTermsPerLaunch];
+    double acc[kTermsPerLaunch];
 #pragma unroll
-    for (uint32_t t = 0; t < kTermsPerLaunch; ++t) accr[t] = acci[t] = 0.0;
+    for (uint32_t t = 0; t < kTermsPerLaunch; ++t) acc[t] = 0.0;
 
     for (uint32_t tl = blk; tl < job.tiles; tl += job.blocks) {
         uint64_t tbase = 0;
@@ -324,36 +337,32 @@ __global__ void __launch_bounds__(kThreads, 5)
             const TileTerm& tt = terms[job.term_begin + t];
             const uint32_t pz = static_cast<uint32_t>(__popcll(tt.z & tbase)) & 1u;
             if (size == (1u << kTileBits))
-                tile_term<true>(tile, tt, pz, u, size, accr[t], acci[t]);
+                tile_term<true>(tile, tt, pz, u, size, acc[t]);
             else
-                tile_term<false>(tile, tt, pz, u, size, accr[t], acci[t]);
+                tile_term<false>(tile, tt, pz, u, size, acc[t]);
         }
     }
-    // Fixed-order tree over the workgroup, one term at a time.
-    for (uint32_t t = 0; t < nt && t < kTermsPerLaunch; ++t) {
-        double vr = 0.0, vi = 0.0;
+    // Fixed order: a butterfly within each wave, then the waves in order.
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
 #pragma unroll
-        for (uint32_t u = 0; u < kTermsPerLaunch; ++u)
-            if (u == t) {
-                vr = accr[u];
-                vi = acci[u];
-            }
-        red[0][threadIdx.x] = vr;
-        red[1][threadIdx.x] = vi;
-        __syncthreads();
-        for (int stride = kThreads / 2; stride > 0; stride >>= 1) {
-            if (threadIdx.x < stride) {
-                red[0][threadIdx.x] = red[0][threadIdx.x] + red[0][threadIdx.x + stride];
-                red[1][threadIdx.x] = red[1][threadIdx.x] + red[1][threadIdx.x + stride];
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            double* pp = partials + terms[job.term_begin + t].partial_off + uint64_t(blk) * 2;
-            pp[0] = red[0][0];
-            pp[1] = red[1][0];
-        }
-        __syncthreads();
+    for (uint32_t t = 0; t < kTermsPerLaunch; ++t) {
+        if (t >= nt) continue;
+        double v = acc[t];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off, 64);
+        if (lane == 0) red[wave][t] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < nt) {
+        const uint32_t t = threadIdx.x;
+        double v = red[0][t];
+#pragma unroll
+        for (uint32_t w = 1; w < kThreads / 64; ++w) v = v + red[w][t];
+        const TileTerm& tt = terms[job.term_begin + t];
+        const double sum = tt.kind == kDiagonalTerm ? v : 2.0 * v;
+        double* pp = partials + tt.partial_off + uint64_t(blk) * 2;
+        pp[0] = tt.kind == kImagTerm ? 0.0 : sum;
+        pp[1] = tt.kind == kImagTerm ? sum : 0.0;
     }
 }
 
@@ -535,6 +544,8 @@ ExpPlan plan_expectation(const std::vector<ExpectationItem>& items) {
                     if ((tt.z >> tj.tpos[i]) & 1ull) tt.zl |= 1u << i;
                 }
                 tt.hl = tt.xl ? 31u - static_cast<uint32_t>(__builtin_clz(tt.xl)) : 0u;
+                tt.kind = tt.x == 0 ? kDiagonalTerm
+                                    : ((__builtin_popcountll(tt.x & tt.z) & 1) ? kImagTerm : kRealTerm);
                 tt.partial_off = partial_doubles;
                 ExpSlot sl;
                 std::memset(&sl, 0, sizeof(sl));
