@@ -7,6 +7,7 @@
 #include <utility>
 #include <vector>
 
+#include "batch_program.h"
 #include "mh_internal.h"
 #include "window.h"
 
@@ -59,6 +60,52 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
                    RoundStats* stats, void* arena = nullptr,
                    const std::function<void(std::size_t, std::size_t)>& prepare = nullptr,
                    const BatchSchedule& schedule = {});
+
+// The term reductions of a batch program, in buckets of states whose passes
+// end at the same launch. A bucket's two reduction launches go on the
+// reduction stream as soon as the launch that finishes its last state is
+// queued (after(), the program's AfterLaunch hook), so the reductions of
+// shallow circuits run beside the passes of deep ones. Within each launch
+// group, states are taken in order of their pass counts into buckets of about
+// half, a third and the last sixth of the group's reduction work (one bucket
+// for small groups). Every term lands at out[2 * (out_index + t)] of a device
+// array; each state's terms take the same kernels, blocks and order as
+// mh_expectation, whatever its bucket.
+class EarlyReductions {
+public:
+    // terms[i]: program item i's terms (terms == 0: none).
+    EarlyReductions(int32_t precision, const BatchProgram& prog, const std::vector<ExpectationItem>& terms,
+                    double* out, hipStream_t reduce_stream);
+    ~EarlyReductions();
+    EarlyReductions(const EarlyReductions&) = delete;
+    EarlyReductions& operator=(const EarlyReductions&) = delete;
+    // Copies every bucket's descriptors on `copy`, not waited for; when
+    // `ordered`, the first reduction waits for the copy on the device (leave
+    // it false when the copy is synchronised before a graph capture).
+    void upload(hipStream_t copy, bool ordered);
+    void after(const BatchLaunch& l, hipStream_t stream);
+    // Queues the buckets no launch finished (states without passes) behind
+    // everything queued so far on `streams`.
+    void flush(const std::vector<hipStream_t>& streams);
+    uint32_t launches() const { return launches_; }
+
+private:
+    struct Bucket {
+        uint32_t group = 0, pass = 0;
+        std::unique_ptr<ExpectationProgram> program;
+        DeviceBuffer workspace;
+        bool queued = false;
+    };
+    void queue(Bucket& b);
+    hipEvent_t event();
+    std::vector<Bucket> buckets_;
+    std::vector<hipEvent_t> events_;
+    hipEvent_t uploaded_ = nullptr;
+    bool wait_upload_ = false;
+    double* out_ = nullptr;
+    hipStream_t reduce_ = nullptr;
+    uint32_t launches_ = 0;
+};
 
 // Runs body(i) for i in [0, count) on up to `threads` host threads; rethrows
 // the first exception.

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <mutex>
@@ -67,6 +68,110 @@ void parallel_for(std::size_t count, uint32_t threads, const std::function<void(
 
 uint64_t round_state_bytes(uint32_t n, int32_t precision) {
     return ((uint64_t{1} << n) * amp_bytes(precision) + kAlign - 1) / kAlign * kAlign;
+}
+
+EarlyReductions::EarlyReductions(int32_t precision, const BatchProgram& prog,
+                                 const std::vector<ExpectationItem>& terms, double* out, hipStream_t reduce_stream)
+    : out_(out), reduce_(reduce_stream) {
+    static constexpr double kCuts[] = {0.5, 0.85, 1.0};
+    static constexpr std::size_t kMinSplit = 48;  // states with terms below which a group keeps one bucket
+    std::map<uint32_t, std::vector<std::size_t>> by_group;
+    for (std::size_t i = 0; i < terms.size(); ++i)
+        if (terms[i].terms > 0) by_group[prog.launch_group_of(i)].push_back(i);
+    for (auto& [group, members] : by_group) {
+        std::stable_sort(members.begin(), members.end(),
+                         [&](std::size_t a, std::size_t b) { return prog.pass_count(a) < prog.pass_count(b); });
+        double total = 0;
+        for (std::size_t i : members) total += std::ldexp(double(terms[i].terms), static_cast<int>(terms[i].n));
+        const std::size_t cuts = members.size() < kMinSplit ? 1 : sizeof(kCuts) / sizeof(kCuts[0]);
+        double done = 0;
+        std::size_t at = 0;
+        for (std::size_t c = 0; c < cuts && at < members.size(); ++c) {
+            const double goal = cuts == 1 ? total : kCuts[c] * total;
+            std::vector<ExpectationItem> bucket;
+            // Whole pass counts at a time: a bucket ends where a pass count ends.
+            while (at < members.size() && (bucket.empty() || done < goal || c + 1 == cuts)) {
+                const uint32_t p = prog.pass_count(members[at]);
+                while (at < members.size() && prog.pass_count(members[at]) == p) {
+                    const ExpectationItem& e = terms[members[at]];
+                    done += std::ldexp(double(e.terms), static_cast<int>(e.n));
+                    bucket.push_back(e);
+                    ++at;
+                }
+            }
+            Bucket b;
+            b.group = group;
+            const uint32_t last = prog.pass_count(members[at - 1]);
+            b.pass = last > 0 ? last - 1 : 0;
+            b.queued = false;
+            b.program = std::make_unique<ExpectationProgram>(precision, bucket);
+            b.workspace = DeviceBuffer(b.program->workspace_bytes());
+            // A bucket of states without passes is queued by flush().
+            if (last == 0) b.pass = UINT32_MAX;
+            buckets_.push_back(std::move(b));
+        }
+    }
+}
+
+EarlyReductions::~EarlyReductions() {
+    for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+    if (uploaded_) (void)hipEventDestroy(uploaded_);
+}
+
+hipEvent_t EarlyReductions::event() {
+    hipEvent_t e = nullptr;
+    MH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    events_.push_back(e);
+    return e;
+}
+
+void EarlyReductions::upload(hipStream_t copy, bool ordered) {
+    for (Bucket& b : buckets_) b.program->upload_on(b.workspace.get(), copy);
+    wait_upload_ = ordered;
+    if (ordered) {
+        MH_HIP(hipEventCreateWithFlags(&uploaded_, hipEventDisableTiming));
+        MH_HIP(hipEventRecord(uploaded_, copy));
+    }
+}
+
+void EarlyReductions::queue(Bucket& b) {
+    if (wait_upload_) {
+        MH_HIP(hipStreamWaitEvent(reduce_, uploaded_, 0));
+        wait_upload_ = false;
+    }
+    b.program->launch_on(b.workspace.get(), reduce_, out_);
+    b.queued = true;
+    launches_ += 2;
+}
+
+void EarlyReductions::after(const BatchLaunch& l, hipStream_t stream) {
+    hipEvent_t done = nullptr;
+    for (Bucket& b : buckets_) {
+        if (b.queued || b.group != l.group || b.pass != l.pass) continue;
+        if (done == nullptr) {
+            done = event();
+            MH_HIP(hipEventRecord(done, stream));
+            MH_HIP(hipStreamWaitEvent(reduce_, done, 0));
+        }
+        queue(b);
+    }
+}
+
+void EarlyReductions::flush(const std::vector<hipStream_t>& streams) {
+    bool waited = false;
+    for (Bucket& b : buckets_) {
+        if (b.queued) continue;
+        if (!waited) {
+            for (hipStream_t st : streams) {
+                if (st == reduce_) continue;
+                hipEvent_t e = event();
+                MH_HIP(hipEventRecord(e, st));
+                MH_HIP(hipStreamWaitEvent(reduce_, e, 0));
+            }
+            waited = true;
+        }
+        queue(b);
+    }
 }
 
 void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t threads, double* terms_out,
@@ -118,28 +223,29 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
     double prepare_seconds = 0;
     double reduce_seconds = 0;
     uint32_t launches = 0;
-    // Each group's reduction, with its workspace; read back after the passes.
-    struct GroupReduction {
-        std::unique_ptr<ExpectationProgram> program;
-        DeviceBuffer workspace;
-        hipEvent_t uploaded = nullptr;
-        ~GroupReduction() {
-            if (uploaded) (void)hipEventDestroy(uploaded);
+    // The round's terms on the device, from its lowest term index; each
+    // planning group's reductions write their own terms there.
+    uint32_t t_lo = UINT32_MAX, t_hi = 0;
+    for (const RoundItem& it : items)
+        if (it.num_terms > 0) {
+            t_lo = std::min(t_lo, it.term_out);
+            t_hi = std::max(t_hi, it.term_out + it.num_terms);
         }
-    };
-    std::vector<std::unique_ptr<GroupReduction>> reductions;
+    DeviceBuffer dterms(t_lo < t_hi ? uint64_t(t_hi - t_lo) * 2 * sizeof(double) : 0);
+    std::vector<std::unique_ptr<EarlyReductions>> reductions;
     {
         // Groups of items: the host plans and compiles group g + 1 while the
         // device runs group g. Group sizes grow (1, 1, 2, 4, 8 sixteenths) so
         // that only a sixteenth of the host work is exposed before the device
         // starts, and every later group's host work is shorter than the device
-        // work queued before it. A group's terms are reduced on the stream its
-        // passes end on, behind them, so the reductions of early groups overlap
-        // the passes of later ones.
+        // work queued before it. A group's terms are reduced in buckets as its
+        // states become final (EarlyReductions), on a stream of their own.
         BatchPassRunner runner(precision, threads, schedule);
+        const hipStream_t reduce_stream = streams().reduce;
+        std::vector<hipStream_t> all_streams{streams().main};
+        for (uint32_t k = 0; k < Streams::kKernelStreams; ++k) all_streams.push_back(streams().kernel(k));
         static constexpr std::size_t kEnds[] = {1, 2, 4, 8, 16};
         const std::size_t groups = count >= 64 ? 5 : 1;
-        reductions.reserve(groups);
         for (std::size_t g = 0; g < groups; ++g) {
             const std::size_t lo = groups == 1 ? 0 : count * (g ? kEnds[g - 1] : 0) / 16;
             const std::size_t hi = groups == 1 ? count : count * kEnds[g] / 16;
@@ -152,40 +258,44 @@ void execute_round(int32_t precision, std::vector<RoundItem>& items, uint32_t th
                 if (items[k].blocks == nullptr) fail(MH_ERR_INTERNAL, "a round item without blocks");
                 passes[k].blocks = items[k].blocks;
             }
-            const hipStream_t tail = runner.add(passes.data() + lo, hi - lo, &bs);
-            const auto r0 = clock::now();
-            std::vector<ExpectationItem> group_terms;
+            std::vector<ExpectationItem> group_terms(hi - lo);
+            bool any_terms = false;
             for (std::size_t k = lo; k < hi; ++k) {
                 const RoundItem& it = items[k];
                 if (it.num_terms == 0) continue;
-                ExpectationItem e;
+                ExpectationItem& e = group_terms[k - lo];
                 e.state = passes[k].state;
                 e.n = it.n;
                 e.terms = it.num_terms;
                 e.x = it.x;
                 e.z = it.z;
-                e.out_index = it.term_out;
-                group_terms.push_back(e);
+                e.out_index = it.term_out - t_lo;
+                any_terms = true;
             }
-            if (!group_terms.empty()) {
-                reductions.push_back(std::make_unique<GroupReduction>());
-                GroupReduction& red = *reductions.back();
-                red.program = std::make_unique<ExpectationProgram>(precision, group_terms);
-                red.workspace = DeviceBuffer(red.program->workspace_bytes());
-                red.program->upload_on(red.workspace.get(), streams().copy);
-                MH_HIP(hipEventCreateWithFlags(&red.uploaded, hipEventDisableTiming));
-                MH_HIP(hipEventRecord(red.uploaded, streams().copy));
-                MH_HIP(hipStreamWaitEvent(tail, red.uploaded, 0));
-                red.program->launch_on(red.workspace.get(), tail);
-                launches += 2;
+            EarlyReductions* red = nullptr;
+            BatchHooks hooks;
+            if (any_terms) {
+                hooks.compiled = [&](const BatchProgram& prog) {
+                    const auto r0 = clock::now();
+                    reductions.push_back(std::make_unique<EarlyReductions>(precision, prog, group_terms,
+                                                                           dterms.as<double>(), reduce_stream));
+                    red = reductions.back().get();
+                    red->upload(streams().copy, true);
+                    reduce_seconds += std::chrono::duration<double>(clock::now() - r0).count();
+                };
+                hooks.after = [&](const BatchLaunch& l, hipStream_t stream) { red->after(l, stream); };
             }
-            reduce_seconds += std::chrono::duration<double>(clock::now() - r0).count();
+            runner.add(passes.data() + lo, hi - lo, &bs, any_terms ? &hooks : nullptr);
+            if (red) {
+                red->flush(all_streams);
+                launches += red->launches();
+            }
         }
         runner.finish();
     }
     const auto p1 = clock::now();
     launches += bs.launches;
-    for (const auto& red : reductions) red->program->read(red->workspace.get(), terms_out);
+    if (t_lo < t_hi) download(terms_out + 2 * uint64_t(t_lo), dterms.get(), dterms.bytes());
     const auto p2 = clock::now();
     for (std::size_t k = 0; k < count; ++k) {
         const RoundItem& it = items[k];

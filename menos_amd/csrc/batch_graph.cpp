@@ -22,6 +22,7 @@
 // matrix table into the pinned staging the graph copies from. A run is one graph
 // launch. Results are the same kernels' as mh_batch_execute's, so terms equal it
 // bit for bit.
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <memory>
@@ -123,10 +124,10 @@ struct mh_batch_graph {
     std::vector<std::vector<mh::Block>> blocks;
     std::vector<mh::BatchItem> items;
     mh::BatchProgram program;
-    std::unique_ptr<mh::ExpectationProgram> expectation;
+    std::unique_ptr<mh::EarlyReductions> reductions;
     std::vector<uint32_t> term_offsets;
     // Device residency.
-    mh::DeviceBuffer arena, bundles, ops, terms, mats, args, groups, workspace;
+    mh::DeviceBuffer arena, bundles, ops, terms, mats, args, groups, dterms;
     uint64_t arena_bytes = 0;
     mh::Pinned pinned_mats, pinned_terms;
     hipGraph_t graph = nullptr;
@@ -192,16 +193,25 @@ void capture(mh_batch_graph& g) {
         // Fork the kernel streams off the main stream, run the program, join.
         MH_HIP(hipEventRecord(g.events[0], main));
         for (uint32_t k = 1; k < nstreams; ++k) MH_HIP(hipStreamWaitEvent(st.kernel(k), g.events[0], 0));
-        g.launches = issue_batch_program(g.program, g.items.data(), dev, g.schedule, 0);
+        // The term reductions on a stream of their own, each bucket behind the
+        // launch that finishes its states (EarlyReductions).
+        const hipStream_t reduce = st.reduce;
+        std::vector<hipStream_t> used{main};
+        for (uint32_t k = 1; k < nstreams; ++k) used.push_back(st.kernel(k));
+        if (g.reductions) MH_HIP(hipStreamWaitEvent(reduce, g.events[0], 0));
+        AfterLaunch after;
+        if (g.reductions) after = [&](const BatchLaunch& l, hipStream_t s) { g.reductions->after(l, s); };
+        g.launches = issue_batch_program(g.program, g.items.data(), dev, g.schedule, 0, after);
+        if (g.reductions) g.reductions->flush(used);
         for (uint32_t k = 1; k < nstreams; ++k) {
             MH_HIP(hipEventRecord(g.events[Streams::kKernelStreams + k], st.kernel(k)));
             MH_HIP(hipStreamWaitEvent(main, g.events[Streams::kKernelStreams + k], 0));
         }
-        if (g.expectation) {
-            g.expectation->launch(g.workspace.get());
-            MH_HIP(hipMemcpyAsync(g.pinned_terms.get(), static_cast<char*>(g.workspace.get()) + g.expectation->out_offset(),
-                                  uint64_t(g.expectation->out_terms()) * 2 * sizeof(double), hipMemcpyDeviceToHost, main));
-            g.launches += 2;
+        if (g.reductions) {
+            MH_HIP(hipEventRecord(g.events[Streams::kKernelStreams], reduce));
+            MH_HIP(hipStreamWaitEvent(main, g.events[Streams::kKernelStreams], 0));
+            MH_HIP(hipMemcpyAsync(g.pinned_terms.get(), g.dterms.get(), g.dterms.bytes(), hipMemcpyDeviceToHost, main));
+            g.launches += g.reductions->launches();
         }
     } catch (...) {
         hipGraph_t partial = nullptr;
@@ -278,7 +288,8 @@ int mh_batch_graph_create(const mh_batch_request* req, mh_batch_graph** out) {
         g->arena_bytes = arena_bytes;
         g->arena = DeviceBuffer(arena_bytes);
         unsigned char* base = static_cast<unsigned char*>(g->arena.get());
-        std::vector<ExpectationItem> exp_items;
+        std::vector<ExpectationItem> exp_items(g->count);
+        bool any_terms = false;
         uint64_t at = 0;
         for (uint32_t i = 0; i < g->count; ++i) {
             BatchItem& it = g->items[i];
@@ -288,14 +299,14 @@ int mh_batch_graph_create(const mh_batch_request* req, mh_batch_graph** out) {
             it.max_cost = req->max_pass_cost;
             const uint32_t nt = req->term_offsets[i + 1] - req->term_offsets[i];
             if (nt > 0) {
-                ExpectationItem e;
+                ExpectationItem& e = exp_items[i];
                 e.state = it.state;
                 e.n = it.n;
                 e.terms = nt;
                 e.x = req->x_masks + req->term_offsets[i];
                 e.z = req->z_masks + req->term_offsets[i];
                 e.out_index = req->term_offsets[i];
-                exp_items.push_back(e);
+                any_terms = true;
             }
             at += round_state_bytes(it.n, req->precision);
         }
@@ -318,11 +329,13 @@ int mh_batch_graph_create(const mh_batch_request* req, mh_batch_graph** out) {
         g->mats = DeviceBuffer(g->program.mats().size());
         g->pinned_mats = Pinned(g->program.mats().size());
         std::memcpy(g->pinned_mats.get(), g->program.mats().data(), g->program.mats().size());
-        if (!exp_items.empty()) {
-            g->expectation = std::make_unique<ExpectationProgram>(req->precision, exp_items);
-            g->workspace = DeviceBuffer(g->expectation->workspace_bytes());
-            g->expectation->upload(g->workspace.get());
-            g->pinned_terms = Pinned(uint64_t(g->expectation->out_terms()) * 2 * sizeof(double));
+        if (any_terms) {
+            g->dterms = DeviceBuffer(uint64_t(g->total_terms) * 2 * sizeof(double));
+            g->reductions = std::make_unique<EarlyReductions>(req->precision, g->program, exp_items,
+                                                              g->dterms.as<double>(), streams().reduce);
+            g->reductions->upload(streams().copy, false);
+            MH_HIP(hipStreamSynchronize(streams().copy));
+            g->pinned_terms = Pinned(g->dterms.bytes());
         }
         synchronize();
         capture(*g);
@@ -412,11 +425,8 @@ int mh_batch_graph_run(mh_batch_graph* g, mh_batch_result* res) {
         context();
         MH_HIP(hipGraphLaunch(g->exec, streams().main));
         synchronize();
-        if (g->expectation) {
-            const double* terms = static_cast<const double*>(g->pinned_terms.get());
-            const uint64_t slots = std::min<uint64_t>(g->total_terms, g->expectation->out_terms());
-            std::memcpy(res->expectation_terms, terms, slots * 2 * sizeof(double));
-        }
+        if (g->reductions)
+            std::memcpy(res->expectation_terms, g->pinned_terms.get(), uint64_t(g->total_terms) * 2 * sizeof(double));
         const double secs = seconds_since(t0);
         if (res->completion_seconds)
             for (uint32_t i = 0; i < g->count; ++i) res->completion_seconds[i] = secs;

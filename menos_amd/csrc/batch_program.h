@@ -6,6 +6,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 #include "mh_internal.h"
@@ -66,6 +67,10 @@ public:
     // sign terms and launch lists (its matrix table alone may differ).
     bool same_shape(const BatchProgram& other) const;
     bool has_generic() const;
+    // Passes of item i, and the launch group it belongs to.
+    uint32_t pass_count(std::size_t item) const;
+    uint32_t launch_group_of(std::size_t item) const;
+    std::size_t item_count() const;
     // Whether some state needs an initialisation kernel (its first pass is not
     // a window pass that can generate it).
     bool needs_initialisation() const;
@@ -84,11 +89,24 @@ private:
 BatchProgram compile_batch_program(int32_t precision, const BatchItem* items, std::size_t count, uint32_t threads,
                                    const BatchSchedule& sched);
 
+// Called after each launch of issue_batch_program (pass l.pass of launch
+// group l.group, with the generic launches of that pass) with the stream it
+// went on: every state of that group with l.pass + 1 passes is then final.
+using AfterLaunch = std::function<void(const BatchLaunch&, hipStream_t)>;
+
 // Issues a compiled program's launches (no waits, no copies, no allocation),
 // launch group g on kernel stream (first_stream + g) mod sched.kernel_streams
 // (everything on the main stream when a pass is generic: such passes launch
 // the generic kernel, which allocates and synchronises). Returns the launch count.
 uint32_t issue_batch_program(const BatchProgram& prog, const BatchItem* items, const BatchDeviceTables& dev,
-                             const BatchSchedule& sched, uint32_t first_stream);
+                             const BatchSchedule& sched, uint32_t first_stream, const AfterLaunch& after = {});
+
+// What BatchPassRunner::add calls around a group's launches: `compiled` once
+// the group's program is built (before its first launch), `after` after each
+// launch (see AfterLaunch).
+struct BatchHooks {
+    std::function<void(const BatchProgram&)> compiled;
+    AfterLaunch after;
+};
 
 }  // namespace mh

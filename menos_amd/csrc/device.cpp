@@ -41,6 +41,7 @@ void create_streams(Streams& s) {
     MH_HIP(hipStreamCreateWithFlags(&s.third, hipStreamNonBlocking));
     MH_HIP(hipStreamCreateWithFlags(&s.fourth, hipStreamNonBlocking));
     MH_HIP(hipStreamCreateWithFlags(&s.copy, hipStreamNonBlocking));
+    MH_HIP(hipStreamCreateWithFlags(&s.reduce, hipStreamNonBlocking));
 }
 
 }  // namespace
@@ -79,7 +80,7 @@ OwnStreams::OwnStreams() {
 
 OwnStreams::~OwnStreams() {
     t_streams = previous_;
-    for (hipStream_t s : {own_.copy, own_.fourth, own_.third, own_.second, own_.main}) {
+    for (hipStream_t s : {own_.reduce, own_.copy, own_.fourth, own_.third, own_.second, own_.main}) {
         (void)hipStreamSynchronize(s);
         (void)hipStreamDestroy(s);
     }

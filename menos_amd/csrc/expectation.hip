@@ -621,12 +621,13 @@ void upload_expectation(const ExpPlan& P, void* ws, hipStream_t stream) {
 }
 
 // The reduction kernels of a plan whose descriptors are in the workspace
-// (on `stream`; no copies).
+// (on `stream`; no copies). The terms land in the workspace's output region,
+// or at out[2 * term] when `out` is given.
 template <typename V>
-void launch_expectation(const ExpPlan& P, void* ws, hipStream_t stream) {
+void launch_expectation(const ExpPlan& P, void* ws, hipStream_t stream, double* out = nullptr) {
     char* w = static_cast<char*>(ws);
     double* partials = reinterpret_cast<double*>(w + P.at_partials);
-    double* dout = reinterpret_cast<double*>(w + P.at_out);
+    double* dout = out ? out : reinterpret_cast<double*>(w + P.at_out);
     if (!P.tjobs.empty()) {
         hipLaunchKernelGGL(expectation_tile_kernel<V>, dim3(P.tile_blocks), dim3(kThreads), 0, stream,
                            reinterpret_cast<const TileJob*>(w + P.at_tjobs),
@@ -691,46 +692,19 @@ ExpectationProgram::ExpectationProgram(int32_t precision, const std::vector<Expe
 ExpectationProgram::~ExpectationProgram() = default;
 
 uint64_t ExpectationProgram::workspace_bytes() const { return std::max<uint64_t>(impl_->plan.bytes, 256); }
-uint64_t ExpectationProgram::out_offset() const { return impl_->plan.at_out; }
-uint32_t ExpectationProgram::out_terms() const { return impl_->plan.out_terms; }
-
-void ExpectationProgram::upload(void* workspace) const {
-    if (impl_->plan.slots.empty()) return;
-    upload_expectation(impl_->plan, workspace, streams().main);
-    synchronize();
-}
 
 void ExpectationProgram::upload_on(void* workspace, hipStream_t stream) const {
     if (impl_->plan.slots.empty()) return;
     upload_expectation(impl_->plan, workspace, stream);
 }
 
-void ExpectationProgram::launch(void* workspace) const { launch_on(workspace, streams().main); }
-
-void ExpectationProgram::launch_on(void* workspace, hipStream_t stream) const {
+void ExpectationProgram::launch_on(void* workspace, hipStream_t stream, double* out) const {
     if (impl_->plan.slots.empty()) return;
     if (impl_->precision == MH_C128)
-        launch_expectation<double2>(impl_->plan, workspace, stream);
+        launch_expectation<double2>(impl_->plan, workspace, stream, out);
     else
-        launch_expectation<float2>(impl_->plan, workspace, stream);
+        launch_expectation<float2>(impl_->plan, workspace, stream, out);
 }
-
-void ExpectationProgram::read(const void* workspace, double* out) const {
-    const ExpPlan& P = impl_->plan;
-    if (P.slots.empty()) return;
-    // Only the slots from the lowest one this program writes.
-    uint64_t lo = P.out_terms;
-    for (const ExpSlot& sl : P.slots) lo = std::min<uint64_t>(lo, sl.out);
-    std::vector<double> host((P.out_terms - lo) * 2);
-    download(host.data(), static_cast<const char*>(workspace) + P.at_out + lo * 2 * sizeof(double),
-             host.size() * sizeof(double));
-    for (const ExpSlot& sl : P.slots) {
-        out[2 * uint64_t(sl.out)] = host[2 * (sl.out - lo)];
-        out[2 * uint64_t(sl.out) + 1] = host[2 * (sl.out - lo) + 1];
-    }
-}
-
-bool ExpectationProgram::empty() const { return impl_->plan.slots.empty(); }
 
 void expectation_batch(int32_t precision, const std::vector<ExpectationItem>& items, double* out) {
     amp_bytes(precision);

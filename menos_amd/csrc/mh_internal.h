@@ -50,6 +50,7 @@ struct Streams {
     hipStream_t third = nullptr;
     hipStream_t fourth = nullptr;
     hipStream_t copy = nullptr;    // table uploads that run ahead of the kernels
+    hipStream_t reduce = nullptr;  // batched term reductions, beside the kernel streams
 
     hipStream_t kernel(uint32_t i) const {
         const hipStream_t k[kKernelStreams] = {main, second, third, fourth};
@@ -176,11 +177,10 @@ struct ExpectationItem {
 };
 void expectation_batch(int32_t precision, const std::vector<ExpectationItem>& items, double* out);
 
-// The same reduction split for a caller that keeps it resident (a captured
-// graph): planned once on the host, its descriptors uploaded once into a
-// workspace of workspace_bytes(), then launched any number of times; each
-// launch leaves term slot t's (re, im) at workspace + out_offset() + 16 t.
-// States must stay where they were when it was planned.
+// The same reduction split for a caller that launches it on streams of its
+// own or keeps it resident (a captured graph): planned once on the host, its
+// descriptors uploaded into a workspace of workspace_bytes(), then launched any
+// number of times. States must stay where they were when it was planned.
 class ExpectationProgram {
 public:
     ExpectationProgram(int32_t precision, const std::vector<ExpectationItem>& items);
@@ -188,20 +188,13 @@ public:
     ExpectationProgram(const ExpectationProgram&) = delete;
     ExpectationProgram& operator=(const ExpectationProgram&) = delete;
     uint64_t workspace_bytes() const;
-    uint64_t out_offset() const;
-    uint32_t out_terms() const;  // slots in the output region (the largest term index + 1)
-    // Copies the descriptors into the workspace on the calling thread's main
-    // stream and waits for the copy.
-    void upload(void* workspace) const;
-    // The same copy on `stream`, not waited for (the program outlives it).
+    // Copies the descriptors into the workspace on `stream`, not waited for
+    // (the program outlives the copy).
     void upload_on(void* workspace, hipStream_t stream) const;
-    // Launches the reduction kernels on the main stream (no copies, no
-    // allocation, no waits: capturable).
-    void launch(void* workspace) const;
-    void launch_on(void* workspace, hipStream_t stream) const;
-    // Copies the terms back (main stream, waited for) into out[2 * term].
-    void read(const void* workspace, double* out) const;
-    bool empty() const;
+    // Launches the reduction kernels on `stream` (no copies, no allocation, no
+    // waits: capturable); term t of item i lands at out[2 * (out_index + t)]
+    // (device memory), or in the workspace's output region when out is null.
+    void launch_on(void* workspace, hipStream_t stream, double* out = nullptr) const;
 
 private:
     struct Impl;

@@ -78,16 +78,19 @@ struct BatchStats {
 // (so the host prepares the next group while the device runs this one).
 // Groups alternate between the two kernel streams, both of which start behind
 // the work queued on the main stream before the runner was made; the groups
-// must touch disjoint states. add() returns the stream on which work queued
-// next runs after every pass of that group (the group's reduction, say).
-// finish() synchronises. The destructor waits for what was launched.
+// must touch disjoint states. `hooks` (optional) sees the group's compiled
+// program and each of its launches. The reduction stream also starts behind
+// the work queued before the runner. finish() synchronises every stream, the
+// reduction stream included. The destructor waits for what was launched.
+struct BatchHooks;  // batch_program.h
+
 class BatchPassRunner {
 public:
     BatchPassRunner(int32_t precision, uint32_t threads, const BatchSchedule& schedule = {});
     ~BatchPassRunner();
     BatchPassRunner(const BatchPassRunner&) = delete;
     BatchPassRunner& operator=(const BatchPassRunner&) = delete;
-    hipStream_t add(const BatchItem* items, std::size_t count, BatchStats* stats);
+    void add(const BatchItem* items, std::size_t count, BatchStats* stats, const BatchHooks* hooks = nullptr);
     void finish();
 
 private:

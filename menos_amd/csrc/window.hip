@@ -255,13 +255,23 @@ __device__ __forceinline__ void xchg(uint32_t& x, uint32_t& y, bool upper) {
         const auto p = __builtin_amdgcn_permlane16_swap(x, y, false, false);
         x = p[0];
         y = p[1];
+    } else if constexpr (I >= 2) {
+        // Lane bits 2 and 3 select banks of four lanes within a row of 16, so
+        // the DPP bank mask picks the lanes that take a partner's value: those
+        // with bit I set take y from lane l - 2^I (row_shr), the others x from
+        // lane l + 2^I (row_shl); a disabled lane keeps its own value.
+        constexpr int shr = I == 2 ? 0x114 : 0x118, shl = I == 2 ? 0x104 : 0x108;
+        constexpr int upper_banks = I == 2 ? 0xA : 0xC, lower_banks = I == 2 ? 0x5 : 0x3;
+        const uint32_t nx = __builtin_amdgcn_update_dpp(x, y, shr, 0xF, upper_banks, false);
+        const uint32_t ny = __builtin_amdgcn_update_dpp(y, x, shl, 0xF, lower_banks, false);
+        (void)upper;
+        x = nx;
+        y = ny;
     } else {
         // Lanes with bit I set take y from lane l - 2^I, the others x from
-        // lane l + 2^I: quad_perm lane ^ 1 (0xB1) or lane ^ 2 (0x4E) for
-        // I < 2, row_shr / row_shl by 4 or 8 otherwise (the partner is in the
-        // same row of 16 lanes).
-        constexpr int from_lower = I == 0 ? 0xB1 : I == 1 ? 0x4E : I == 2 ? 0x114 : 0x118;
-        constexpr int from_upper = I == 0 ? 0xB1 : I == 1 ? 0x4E : I == 2 ? 0x104 : 0x108;
+        // lane l + 2^I: quad_perm lane ^ 1 (0xB1) or lane ^ 2 (0x4E).
+        constexpr int from_lower = I == 0 ? 0xB1 : 0x4E;
+        constexpr int from_upper = I == 0 ? 0xB1 : 0x4E;
         const uint32_t fy = __builtin_amdgcn_mov_dpp(y, from_lower, 0xF, 0xF, true);
         const uint32_t fx = __builtin_amdgcn_mov_dpp(x, from_upper, 0xF, 0xF, true);
         const uint32_t nx = upper ? fy : x;
@@ -1240,12 +1250,9 @@ struct BatchKeep {
     std::vector<DeviceBuffer> device;
     std::vector<std::shared_ptr<void>> host;
     hipEvent_t copied = nullptr;
-    std::vector<hipEvent_t> joins;  // launch groups' ends, waited for on the tail stream
     void release() {
         if (copied) (void)hipEventDestroy(copied);
         copied = nullptr;
-        for (hipEvent_t e : joins) (void)hipEventDestroy(e);
-        joins.clear();
     }
 };
 
@@ -1291,6 +1298,14 @@ const std::vector<unsigned char>& BatchProgram::mats() const { return d_->mats; 
 const std::vector<unsigned char>& BatchProgram::args() const { return d_->args; }
 const std::vector<unsigned char>& BatchProgram::groups() const { return d_->groups; }
 bool BatchProgram::has_generic() const { return d_->has_generic; }
+uint32_t BatchProgram::pass_count(std::size_t item) const { return static_cast<uint32_t>(d_->passes[item].size()); }
+uint32_t BatchProgram::launch_group_of(std::size_t item) const {
+    for (uint32_t lg = 0; lg < d_->lgroups.size(); ++lg)
+        for (std::size_t i : d_->lgroups[lg])
+            if (i == item) return lg;
+    return 0;
+}
+std::size_t BatchProgram::item_count() const { return d_->passes.size(); }
 bool BatchProgram::needs_initialisation() const { return !d_->needs_init.empty(); }
 uint32_t BatchProgram::launch_count() const { return static_cast<uint32_t>(d_->launches.size()); }
 BatchProgramData& BatchProgram::data() { return *d_; }
@@ -1503,7 +1518,7 @@ void compile_batch_typed(const BatchItem* items, std::size_t count, uint32_t thr
 
 template <typename R>
 uint32_t issue_batch_typed(const BatchProgramData& prog, const BatchItem* items, const BatchDeviceTables& dev,
-                           const BatchSchedule& sched, uint32_t first_stream) {
+                           const BatchSchedule& sched, uint32_t first_stream, const AfterLaunch& after) {
     using V = typename Shape<R>::V;
     uintptr_t base_addr = UINTPTR_MAX;
     for (std::size_t i = 0; i < prog.passes.size(); ++i)
@@ -1555,6 +1570,7 @@ uint32_t issue_batch_typed(const BatchProgramData& prog, const BatchItem* items,
                            static_cast<uint32_t>(blk.operands.size()));
             ++launches;
         }
+        if (after) after(l, stream);
     }
     return launches;
 }
@@ -1572,12 +1588,10 @@ void keep_table(BatchKeep& keep, const std::vector<unsigned char>& src, hipStrea
 // Many states, each with its own block list, compiled into one program whose
 // tables go up on the copy stream; its launches then wait for them on the
 // kernel streams (see issue_batch_program). Returns without waiting; `keep`
-// holds what the launches read. Returns the stream on which work queued next
-// runs after every pass of these items (the main stream when a pass is
-// generic, else the first launch group's stream, made to wait for the others).
-hipStream_t launch_window_passes_batch(int32_t precision, const BatchItem* items, std::size_t count, uint32_t threads,
-                                       BatchStats* stats, BatchKeep& keep, const BatchSchedule& sched,
-                                       uint32_t first_stream) {
+// holds what the launches read.
+void launch_window_passes_batch(int32_t precision, const BatchItem* items, std::size_t count, uint32_t threads,
+                                BatchStats* stats, BatchKeep& keep, const BatchSchedule& sched, uint32_t first_stream,
+                                const BatchHooks* hooks) {
     context();
     const auto c0 = std::chrono::steady_clock::now();
     const BatchProgram prog = compile_batch_program(precision, items, count, threads, sched);
@@ -1603,23 +1617,13 @@ hipStream_t launch_window_passes_batch(int32_t precision, const BatchItem* items
         for (uint32_t k = 0; k < nstreams; ++k) MH_HIP(hipStreamWaitEvent(streams().kernel(k), keep.copied, 0));
     }
     if (stats) stats->compile_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
-    const uint32_t launches = issue_batch_program(prog, items, dev, sched, first_stream);
+    if (hooks && hooks->compiled) hooks->compiled(prog);
+    const uint32_t launches =
+        issue_batch_program(prog, items, dev, sched, first_stream, hooks ? hooks->after : AfterLaunch{});
     if (stats) {
         stats->launches += launches;
         stats->max_passes = std::max(stats->max_passes, prog.data().max_passes);
     }
-    if (prog.has_generic()) return streams().main;
-    const uint32_t nstreams = std::max(1u, std::min(sched.kernel_streams, Streams::kKernelStreams));
-    const hipStream_t tail = streams().kernel(first_stream % nstreams);
-    const std::size_t used = std::min<std::size_t>(prog.data().lgroups.size(), nstreams);
-    for (std::size_t g = 1; g < used; ++g) {
-        hipEvent_t e = nullptr;
-        MH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        keep.joins.push_back(e);
-        MH_HIP(hipEventRecord(e, streams().kernel((first_stream + g) % nstreams)));
-        MH_HIP(hipStreamWaitEvent(tail, e, 0));
-    }
-    return tail;
 }
 
 }  // namespace
@@ -1635,9 +1639,10 @@ BatchProgram compile_batch_program(int32_t precision, const BatchItem* items, st
 }
 
 uint32_t issue_batch_program(const BatchProgram& prog, const BatchItem* items, const BatchDeviceTables& dev,
-                             const BatchSchedule& sched, uint32_t first_stream) {
-    if (prog.precision() == MH_C128) return issue_batch_typed<double>(prog.data(), items, dev, sched, first_stream);
-    return issue_batch_typed<float>(prog.data(), items, dev, sched, first_stream);
+                             const BatchSchedule& sched, uint32_t first_stream, const AfterLaunch& after) {
+    if (prog.precision() == MH_C128)
+        return issue_batch_typed<double>(prog.data(), items, dev, sched, first_stream, after);
+    return issue_batch_typed<float>(prog.data(), items, dev, sched, first_stream, after);
 }
 
 uint32_t window_width(int32_t precision) noexcept {
@@ -1717,6 +1722,7 @@ BatchPassRunner::BatchPassRunner(int32_t precision, uint32_t threads, const Batc
     MH_HIP(hipEventCreateWithFlags(&impl_->start, hipEventDisableTiming));
     MH_HIP(hipEventRecord(impl_->start, streams().main));
     for (uint32_t k = 1; k < Streams::kKernelStreams; ++k) MH_HIP(hipStreamWaitEvent(streams().kernel(k), impl_->start, 0));
+    MH_HIP(hipStreamWaitEvent(streams().reduce, impl_->start, 0));
 }
 
 BatchPassRunner::~BatchPassRunner() {
@@ -1724,13 +1730,14 @@ BatchPassRunner::~BatchPassRunner() {
     if (!impl_->keeps.empty()) {
         (void)hipStreamSynchronize(streams().copy);
         for (uint32_t k = 0; k < Streams::kKernelStreams; ++k) (void)hipStreamSynchronize(streams().kernel(k));
+        (void)hipStreamSynchronize(streams().reduce);
     }
     for (BatchKeep& k : impl_->keeps) k.release();
     if (impl_->start) (void)hipEventDestroy(impl_->start);
 }
 
-hipStream_t BatchPassRunner::add(const BatchItem* items, std::size_t count, BatchStats* stats) {
-    if (count == 0) return streams().main;
+void BatchPassRunner::add(const BatchItem* items, std::size_t count, BatchStats* stats, const BatchHooks* hooks) {
+    if (count == 0) return;
     const uint32_t width = window_width(impl_->precision);
     for (std::size_t i = 0; i < count; ++i)
         if (items[i].n < width) fail(MH_ERR_INVALID, "a batched state has fewer qubits than one window");
@@ -1739,13 +1746,14 @@ hipStream_t BatchPassRunner::add(const BatchItem* items, std::size_t count, Batc
     // the end of one call overlap the next call's.
     context();
     const uint32_t first_stream = impl_->added++;
-    return launch_window_passes_batch(impl_->precision, items, count, impl_->threads, stats, impl_->keeps.back(),
-                               impl_->schedule, first_stream);
+    launch_window_passes_batch(impl_->precision, items, count, impl_->threads, stats, impl_->keeps.back(),
+                               impl_->schedule, first_stream, hooks);
 }
 
 void BatchPassRunner::finish() {
     MH_HIP(hipStreamSynchronize(streams().copy));
     for (uint32_t k = 1; k < Streams::kKernelStreams; ++k) MH_HIP(hipStreamSynchronize(streams().kernel(k)));
+    MH_HIP(hipStreamSynchronize(streams().reduce));
     synchronize();
     for (BatchKeep& k : impl_->keeps) k.release();
     impl_->keeps.clear();
