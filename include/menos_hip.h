@@ -258,7 +258,19 @@ typedef struct mh_execution_request {
     uint32_t max_pass_cost;        /* FP operations per amplitude one HBM pass may carry; 0 = default */
     mh_monitor_fn monitor;         /* null: no monitor */
     void* monitor_data;
+    int32_t want_region_sampler;   /* keep the final state on the device as a region sampler (below) */
+    uint32_t reserved0;
 } mh_execution_request;
+
+/* A device-resident region sampler: the reference's CumulativeDistribution
+ * (src/engines/dense/cumulative_distribution.cpp:14-92), the sampler a
+ * non-sole region hands the runtime (RegionCountsSampler,
+ * include/menos/simulation_engine.h:47-52; runtime/combine_counts.cpp:10-36),
+ * over a state that stays in device memory instead of a host copy of its 2^n
+ * amplitudes. mh_engine_execute with want_region_sampler set keeps its final
+ * state and returns the sampler in result->region_sampler; the caller owns it
+ * and frees it (and the state) with mh_region_sampler_destroy. */
+typedef struct mh_region_sampler mh_region_sampler;
 
 typedef struct mh_execution_result {
     /* Caller-owned outputs, any may be null when not requested. */
@@ -282,6 +294,7 @@ typedef struct mh_execution_result {
     double expectation_seconds;
     int32_t fusion_used;           /* the plan run: the request's, or the autotuner's choice */
     uint32_t fusion_width_used;
+    mh_region_sampler* region_sampler; /* want_region_sampler: the caller's to destroy; else null */
 } mh_execution_result;
 
 typedef struct mh_cost_estimate {
@@ -309,6 +322,18 @@ typedef struct mh_cost_estimate {
  *   + terms * state / hbm + h2d / h2d_rate + d2h / d2h_rate. */
 int mh_engine_estimate(const mh_execution_request* request, mh_cost_estimate* out);
 int mh_engine_execute(const mh_execution_request* request, mh_execution_result* result);
+
+/* Basis indices for `count` uniforms in [0, 1), in any order: index i is the
+ * first basis state whose running sum fl(c + |a|^2) in index order exceeds
+ * uniforms[i] (the last one if none), as CumulativeDistribution::sample_one
+ * (cumulative_distribution.cpp:83-92) finds it, bit for bit. Reads only the
+ * sampler's device tables; calls on one sampler may run from several threads.
+ * A uniform outside [0, 1) is MH_ERR_INVALID. */
+int mh_region_sampler_draw(const mh_region_sampler* sampler, const double* uniforms, uint64_t count,
+                           uint64_t* basis_out);
+/* The state's qubit count and the device bytes the sampler holds. */
+int mh_region_sampler_info(const mh_region_sampler* sampler, uint32_t* num_qubits, uint64_t* device_bytes);
+int mh_region_sampler_destroy(mh_region_sampler* sampler);
 
 /* The fusion plan the engine would run (FusionPlan layout, reference
  * src/ir/fuse_gates.h:38-76). Outputs sized for the worst case by the caller:

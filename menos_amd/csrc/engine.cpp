@@ -285,6 +285,8 @@ ExecutionResult HipDenseEngine::execute(const ExecutionRequest& r) const {
         }
         result.actual_bytes = sb;
         result.peak_device_bytes = std::max(ledger.peak(), sb);
+        // Last, so that nothing after it can fail and leak the state.
+        if (r.want_region_sampler) result.region_sampler = make_region_sampler(state.release(), n, r.precision);
     }
     result.execute_seconds = std::chrono::duration<double>(clock::now() - t0).count();
     return result;
@@ -316,6 +318,7 @@ ExecutionRequest to_request(const mh_execution_request* q) {
     r.max_pass_cost = q->max_pass_cost;
     r.monitor = q->monitor;
     r.monitor_data = q->monitor_data;
+    r.want_region_sampler = q->want_region_sampler != 0;
     return r;
 }
 
@@ -353,8 +356,10 @@ int mh_engine_estimate(const mh_execution_request* request, mh_cost_estimate* ou
 int mh_engine_execute(const mh_execution_request* request, mh_execution_result* out) {
     return guarded([&] {
         if (out == nullptr) fail(MH_ERR_INVALID, "null result");
+        out->region_sampler = nullptr;
         const ExecutionRequest r = to_request(request);
         ExecutionResult res = HipDenseEngine{}.execute(r);
+        out->region_sampler = res.region_sampler;
         if (res.statevector && out->statevector) std::memcpy(out->statevector, res.statevector->data(), res.statevector->size());
         out->num_outcomes = 0;
         if (res.counts && out->outcomes && out->counts) {

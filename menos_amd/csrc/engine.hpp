@@ -44,6 +44,12 @@ public:
     ChargedBuffer(const ChargedBuffer&) = delete;
     ChargedBuffer& operator=(const ChargedBuffer&) = delete;
     void* get() const { return buf_.get(); }
+    // The memory, handed over: no longer charged to this call's ledger.
+    DeviceBuffer release() {
+        DeviceLedger::current().release(bytes_);
+        bytes_ = 0;
+        return std::move(buf_);
+    }
 
 private:
     DeviceBuffer buf_;
@@ -68,6 +74,7 @@ struct ExecutionRequest {
     uint32_t max_pass_cost = 0;
     mh_monitor_fn monitor = nullptr;
     void* monitor_data = nullptr;
+    bool want_region_sampler = false;
 };
 
 struct ExecutionResult {
@@ -83,6 +90,7 @@ struct ExecutionResult {
     double execute_seconds = 0, gate_seconds = 0, sample_seconds = 0, expectation_seconds = 0;
     int32_t fusion_used = MH_FUSE_CONTAINMENT;
     uint32_t width_used = 0;
+    mh_region_sampler* region_sampler = nullptr;  // want_region_sampler: the caller's to destroy
 };
 
 struct CostEstimate {

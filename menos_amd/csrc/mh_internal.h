@@ -204,6 +204,9 @@ private:
 // ---- sampler (sampler.hip)
 void sample_sorted(const void* state, uint32_t n, int32_t precision, const double* sorted_u, uint64_t shots,
                    uint64_t* basis_out);
+// Keeps `state` (2^n amplitudes) on the device with its sampler tables
+// (mh_region_sampler_*); synchronises.
+mh_region_sampler* make_region_sampler(DeviceBuffer&& state, uint32_t n, int32_t precision);
 
 // ---- host IR (host_ir.cpp)
 struct CircuitView {

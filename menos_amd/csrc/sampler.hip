@@ -406,24 +406,32 @@ __global__ void __launch_bounds__(kThreads) shots_kernel(SampleArgs a, const dou
     out[t] = hit;
 }
 
-template <typename V>
-void run_sampler(const void* state, uint32_t n, const double* sorted_u, uint64_t shots, uint64_t* basis_out) {
-    context();
+// The first half of the sampler over one state: the exact running value at
+// every block start (block sums, predicted binades, composed transfers, the
+// chain, the starts inside jumped groups), in device tables kept for draws.
+struct SamplerTables {
+    DeviceBuffer buf;
     SampleArgs a{};
+};
+
+uint64_t r16(uint64_t bytes) { return (bytes + 15) / 16 * 16; }
+
+template <typename V>
+void prepare_tables(SamplerTables& t, const void* state, uint32_t n) {
+    context();
+    SampleArgs& a = t.a;
     a.state = state;
     a.size = uint64_t{1} << n;
     a.block = uint64_t{1} << std::min<uint32_t>(n, kBlockLog);
     a.blocks = static_cast<uint32_t>(a.size / a.block);
     a.groups = (a.blocks + kGroup - 1) / kGroup;
     const uint64_t nb = a.blocks;
-    auto r16 = [](uint64_t bytes) { return (bytes + 15) / 16 * 16; };
-    DeviceBuffer buf(r16(nb * sizeof(Xfer)) + r16(uint64_t(a.groups) * sizeof(Xfer)) + r16(nb * sizeof(double)) +
-                     2 * r16((nb + 1) * sizeof(double)) + r16(uint64_t(a.groups) * sizeof(uint32_t)) +
-                     r16(shots * sizeof(double)) + r16(shots * sizeof(uint64_t)));
-    char* p = buf.as<char>();
+    t.buf = DeviceBuffer(r16(nb * sizeof(Xfer)) + r16(uint64_t(a.groups) * sizeof(Xfer)) + r16(nb * sizeof(double)) +
+                         2 * r16((nb + 1) * sizeof(double)) + r16(uint64_t(a.groups) * sizeof(uint32_t)));
+    char* p = t.buf.as<char>();
     auto take = [&](uint64_t bytes) {
         char* r = p;
-        p += (bytes + 15) / 16 * 16;
+        p += r16(bytes);
         return r;
     };
     a.xfer = reinterpret_cast<Xfer*>(take(nb * sizeof(Xfer)));
@@ -432,10 +440,6 @@ void run_sampler(const void* state, uint32_t n, const double* sorted_u, uint64_t
     a.approx = reinterpret_cast<double*>(take((nb + 1) * sizeof(double)));
     a.cstart = reinterpret_cast<double*>(take((nb + 1) * sizeof(double)));
     a.jumped = reinterpret_cast<uint32_t*>(take(uint64_t(a.groups) * sizeof(uint32_t)));
-    double* du = reinterpret_cast<double*>(take(shots * sizeof(double)));
-    uint64_t* dout = reinterpret_cast<uint64_t*>(take(shots * sizeof(uint64_t)));
-    upload(du, sorted_u, shots * sizeof(double));
-
     hipLaunchKernelGGL(block_sums_kernel<V>, dim3(a.blocks), dim3(kThreads), 0, streams().main, a);
     MH_HIP(hipGetLastError());
     hipLaunchKernelGGL(approx_prefix_kernel, dim3(1), dim3(1024), 0, streams().main, a);
@@ -448,13 +452,29 @@ void run_sampler(const void* state, uint32_t n, const double* sorted_u, uint64_t
     MH_HIP(hipGetLastError());
     hipLaunchKernelGGL(expand_kernel, dim3(a.groups), dim3(kThreads), 0, streams().main, a);
     MH_HIP(hipGetLastError());
-    if (shots > 0) {
-        const uint32_t grid = static_cast<uint32_t>((shots + kThreads - 1) / kThreads);
-        hipLaunchKernelGGL(shots_kernel<V>, dim3(grid), dim3(kThreads), 0, streams().main, a,
-                           static_cast<const double*>(du), shots, dout);
-        MH_HIP(hipGetLastError());
-    }
-    download(basis_out, dout, shots * sizeof(uint64_t));
+}
+
+// The second half: each uniform's basis index (any order), waited for.
+template <typename V>
+void draw_basis(const SampleArgs& a, const double* uniforms, uint64_t count, uint64_t* basis_out) {
+    if (count == 0) return;
+    context();
+    DeviceBuffer io(2 * r16(count * sizeof(double)));
+    double* du = io.as<double>();
+    uint64_t* dout = reinterpret_cast<uint64_t*>(io.as<char>() + r16(count * sizeof(double)));
+    upload(du, uniforms, count * sizeof(double));
+    const uint32_t grid = static_cast<uint32_t>((count + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(shots_kernel<V>, dim3(grid), dim3(kThreads), 0, streams().main, a,
+                       static_cast<const double*>(du), count, dout);
+    MH_HIP(hipGetLastError());
+    download(basis_out, dout, count * sizeof(uint64_t));
+}
+
+template <typename V>
+void run_sampler(const void* state, uint32_t n, const double* sorted_u, uint64_t shots, uint64_t* basis_out) {
+    SamplerTables t;
+    prepare_tables<V>(t, state, n);
+    draw_basis<V>(t.a, sorted_u, shots, basis_out);
 }
 
 }  // namespace
@@ -474,6 +494,33 @@ void sample_sorted(const void* state, uint32_t n, int32_t precision, const doubl
 
 }  // namespace mh
 
+// A final state kept on the device with its sampler tables.
+struct mh_region_sampler {
+    mh::DeviceBuffer state;
+    mh::SamplerTables tables;
+    uint32_t n = 0;
+    int32_t precision = MH_C128;
+};
+
+namespace mh {
+
+mh_region_sampler* make_region_sampler(DeviceBuffer&& state, uint32_t n, int32_t precision) {
+    amp_bytes(precision);
+    if (n > 40) fail(MH_ERR_INVALID, "region sampler: num_qubits above 40");
+    auto s = std::make_unique<mh_region_sampler>();
+    s->state = std::move(state);
+    s->n = n;
+    s->precision = precision;
+    if (precision == MH_C128)
+        prepare_tables<double2>(s->tables, s->state.get(), n);
+    else
+        prepare_tables<float2>(s->tables, s->state.get(), n);
+    synchronize();
+    return s.release();
+}
+
+}  // namespace mh
+
 using namespace mh;
 
 extern "C" {
@@ -481,6 +528,37 @@ extern "C" {
 int mh_sample(const void* state, uint32_t num_qubits, int32_t precision, const double* sorted_uniforms,
               uint64_t shots, uint64_t* basis_out) {
     return guarded([&] { sample_sorted(state, num_qubits, precision, sorted_uniforms, shots, basis_out); });
+}
+
+int mh_region_sampler_draw(const mh_region_sampler* sampler, const double* uniforms, uint64_t count,
+                           uint64_t* basis_out) {
+    return guarded([&] {
+        if (sampler == nullptr) fail(MH_ERR_INVALID, "null region sampler");
+        if (count == 0) return;
+        if (uniforms == nullptr || basis_out == nullptr) fail(MH_ERR_INVALID, "null uniforms or output");
+        for (uint64_t i = 0; i < count; ++i)
+            if (!(uniforms[i] >= 0.0 && uniforms[i] < 1.0)) fail(MH_ERR_INVALID, "region sampler: a uniform outside [0, 1)");
+        if (sampler->precision == MH_C128)
+            draw_basis<double2>(sampler->tables.a, uniforms, count, basis_out);
+        else
+            draw_basis<float2>(sampler->tables.a, uniforms, count, basis_out);
+    });
+}
+
+int mh_region_sampler_info(const mh_region_sampler* sampler, uint32_t* num_qubits, uint64_t* device_bytes) {
+    return guarded([&] {
+        if (sampler == nullptr) fail(MH_ERR_INVALID, "null region sampler");
+        if (num_qubits) *num_qubits = sampler->n;
+        if (device_bytes) *device_bytes = sampler->state.bytes() + sampler->tables.buf.bytes();
+    });
+}
+
+int mh_region_sampler_destroy(mh_region_sampler* sampler) {
+    return guarded([&] {
+        if (sampler == nullptr) return;
+        synchronize();
+        delete sampler;
+    });
 }
 
 int mh_sample_unsorted(const void* state, uint32_t num_qubits, int32_t precision, const double* uniforms,

@@ -34,6 +34,7 @@ EXPORTS = (
     "mh_derive_seeds", "mh_set_calibration", "mh_get_calibration", "mh_expectation_noalloc",
     "mh_plan_passes", "mh_gather", "mh_batch_graph_create", "mh_batch_graph_update", "mh_batch_graph_run",
     "mh_batch_graph_destroy", "mh_batch_graph_set_params", "mh_batch_graph_param_count",
+    "mh_region_sampler_draw", "mh_region_sampler_info", "mh_region_sampler_destroy",
 )
 JOB_ESTIMATOR, JOB_SAMPLER = 0, 1
 
@@ -75,7 +76,8 @@ class CRequest(ctypes.Structure):
                 ("bitstring_width", ctypes.c_uint32), ("num_terms", ctypes.c_uint32), ("x_masks", _u64p),
                 ("z_masks", _u64p), ("budget_bytes", ctypes.c_uint64), ("want_statevector", ctypes.c_int32),
                 ("record_pass_times", ctypes.c_int32), ("max_pass_cost", ctypes.c_uint32),
-                ("monitor", MONITOR_FN), ("monitor_data", ctypes.c_void_p)]
+                ("monitor", MONITOR_FN), ("monitor_data", ctypes.c_void_p),
+                ("want_region_sampler", ctypes.c_int32), ("reserved0", ctypes.c_uint32)]
 
 
 class CResult(ctypes.Structure):
@@ -87,7 +89,7 @@ class CResult(ctypes.Structure):
                 ("peak_device_bytes", ctypes.c_uint64), ("execute_seconds", ctypes.c_double),
                 ("gate_seconds", ctypes.c_double), ("sample_seconds", ctypes.c_double),
                 ("expectation_seconds", ctypes.c_double), ("fusion_used", ctypes.c_int32),
-                ("fusion_width_used", ctypes.c_uint32)]
+                ("fusion_width_used", ctypes.c_uint32), ("region_sampler", ctypes.c_void_p)]
 
 
 class CCost(ctypes.Structure):
@@ -343,10 +345,12 @@ class HipDenseEngine:
     def execute(self, circuit: W.Circuit, *, precision=C128, fusion=FUSE_CONTAINMENT, fusion_width=0,
                 window_qubits=0, shots=0, seed=0, targets=None, width=None, terms=(), budget_bytes=0,
                 statevector=True, record_pass_times=False, max_passes=4096, max_pass_cost=0,
-                monitor=None) -> Result:
+                monitor=None, region_sampler=False) -> Result:
         """``monitor(boundaries_crossed)`` runs after every HBM pass; it
         abandons the run by raising (as the reference's at_safe_boundary
-        does), which this call re-raises after the engine has freed the state."""
+        does), which this call re-raises after the engine has freed the state.
+        ``region_sampler``: the final state stays on the device as
+        ``Result.region_sampler`` (a RegionSampler)."""
         keep = _CircuitKeep(circuit)
         req, hold = self._request(circuit, keep, precision=precision, fusion=fusion, fusion_width=fusion_width,
                                   window_qubits=window_qubits, shots=shots, seed=seed, targets=targets, width=width,
@@ -362,6 +366,7 @@ class HipDenseEngine:
                     raised.append(e)
                     return 1
             req.monitor = MONITOR_FN(_boundary)
+        req.want_region_sampler = int(bool(region_sampler))
         sv = np.zeros(1 << circuit.num_qubits, amp_dtype(precision)) if statevector else None
         outc = np.zeros(max(shots, 1), np.uint64)
         cnt = np.zeros(max(shots, 1), np.uint64)
@@ -383,7 +388,46 @@ class HipDenseEngine:
                       peak_device_bytes=res.peak_device_bytes, execute_seconds=res.execute_seconds,
                       gate_seconds=res.gate_seconds, sample_seconds=res.sample_seconds,
                       expectation_seconds=res.expectation_seconds, fusion_used=res.fusion_used,
-                      fusion_width_used=res.fusion_width_used)
+                      fusion_width_used=res.fusion_width_used,
+                      region_sampler=RegionSampler(res.region_sampler) if res.region_sampler else None)
+
+
+class RegionSampler:
+    """A final state kept on the device with its cumulative-distribution tables
+    (mh_region_sampler_*): the reference's CumulativeDistribution without the
+    host copy."""
+
+    def __init__(self, handle):
+        self.ptr = ctypes.c_void_p(handle)
+        n = ctypes.c_uint32()
+        nbytes = ctypes.c_uint64()
+        check(lib().mh_region_sampler_info(self.ptr, ctypes.byref(n), ctypes.byref(nbytes)))
+        self.num_qubits = n.value
+        self.device_bytes = nbytes.value
+
+    def draw(self, uniforms) -> np.ndarray:
+        """The basis index of each uniform in [0, 1), in the given order."""
+        u = np.ascontiguousarray(uniforms, np.float64)
+        out = np.zeros(max(u.size, 1), np.uint64)
+        check(lib().mh_region_sampler_draw(self.ptr, _ptr(u, _f64p), u.size, out.ctypes.data_as(_u64p)))
+        return out[:u.size]
+
+    def close(self):
+        if self.ptr:
+            check(lib().mh_region_sampler_destroy(self.ptr))
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class CBatchRequest(ctypes.Structure):

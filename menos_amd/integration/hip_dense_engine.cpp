@@ -2,6 +2,7 @@
 // Reference counterpart: CudaDenseEngine, src/engines/dense_cuda/cuda_dense_engine.cpp:182-412.
 #include "hip_dense_engine.h"
 
+#include <algorithm>
 #include <complex>
 #include <exception>
 #include <map>
@@ -14,7 +15,7 @@
 
 #include "common/error.h"
 #include "common/memory_ledger.h"
-#include "engines/dense/cumulative_distribution.h"
+#include "common/rng.h"
 #include "ir/noise.h"
 
 namespace menos::engines::dense_hip {
@@ -104,6 +105,81 @@ BitString bitstring_of(std::uint64_t word, std::uint32_t width) {
     return key;
 }
 
+// A non-sole region's sampler over the final state kept on the device
+// (mh_region_sampler_*), in place of the reference's CumulativeDistribution over
+// a host copy (cumulative_distribution.cpp:14-92; the CUDA engine copies the
+// state back for it, cuda_dense_engine.cpp:386-399). sample_one draws exactly
+// one word from the region's stream per call, as the reference does, and
+// returns the same outcome bit for bit. Draws go to the device in batches: on
+// the first call (and whenever a batch is used up) a copy of the caller's Rng
+// runs ahead for the shots the request asked for, and each call then checks
+// that its own word is the one predicted (a caller that draws from the stream
+// between calls gets a single-word draw instead). The device memory is charged
+// to the ledger of the thread that executed the region while the sampler lives.
+class HipRegionSampler final : public RegionCountsSampler {
+public:
+    HipRegionSampler(mh_region_sampler* sampler, std::span<const std::pair<std::uint32_t, std::uint32_t>> targets,
+                     std::uint32_t width, std::uint64_t expected_draws)
+        : sampler_(sampler), targets_(targets.begin(), targets.end()), width_(width),
+          expected_(std::max<std::uint64_t>(expected_draws, 1)) {
+        std::uint64_t bytes = 0;
+        if (mh_region_sampler_info(sampler_, nullptr, &bytes) == MH_OK)
+            charge_.emplace(bytes, common::MemoryLedger::Space::Device);
+    }
+    ~HipRegionSampler() override { (void)mh_region_sampler_destroy(sampler_); }
+    HipRegionSampler(const HipRegionSampler&) = delete;
+    HipRegionSampler& operator=(const HipRegionSampler&) = delete;
+
+    [[nodiscard]] BitString sample_one(common::Rng& rng) const override {
+        if (next_ == words_.size()) refill(rng);
+        const std::uint64_t word = rng.next_word();
+        std::uint64_t basis = 0;
+        if (word == words_[next_]) {
+            basis = basis_[next_++];
+        } else {
+            words_.clear();
+            basis_.clear();
+            next_ = 0;
+            const double u = common::Rng::word_to_uniform(word);
+            if (const int st = mh_region_sampler_draw(sampler_, &u, 1, &basis); st != MH_OK)
+                throw_status(st, "HipRegionSampler::sample_one");
+        }
+        ++drawn_;
+        BitString outcome(width_);
+        for (const auto& [qubit, clbit] : targets_)
+            if ((basis >> qubit) & 1u) outcome.set(clbit, true);
+        return outcome;
+    }
+
+private:
+    static constexpr std::uint64_t kMaxBatch = std::uint64_t{1} << 20;
+
+    void refill(const common::Rng& rng) const {
+        const std::uint64_t left = expected_ > drawn_ ? expected_ - drawn_ : expected_;
+        const std::uint64_t count = std::min(left, kMaxBatch);
+        common::Rng ahead = rng;
+        words_.resize(count);
+        std::vector<double> u(count);
+        for (std::uint64_t i = 0; i < count; ++i) {
+            words_[i] = ahead.next_word();
+            u[i] = common::Rng::word_to_uniform(words_[i]);
+        }
+        basis_.resize(count);
+        if (const int st = mh_region_sampler_draw(sampler_, u.data(), count, basis_.data()); st != MH_OK)
+            throw_status(st, "HipRegionSampler::sample_one");
+        next_ = 0;
+    }
+
+    mh_region_sampler* sampler_;
+    std::vector<std::pair<std::uint32_t, std::uint32_t>> targets_;
+    std::uint32_t width_;
+    std::uint64_t expected_;
+    std::optional<common::MemoryLedger::Charge> charge_;
+    mutable std::vector<std::uint64_t> words_, basis_;
+    mutable std::size_t next_ = 0;
+    mutable std::uint64_t drawn_ = 0;
+};
+
 }  // namespace
 
 bool hip_device_available() {
@@ -166,7 +242,7 @@ CostEstimate HipDenseEngine::estimate(const EngineEstimateRequest& request) cons
         q.precision = MH_C128;
         q.fusion = request.fuse_gates ? MH_FUSE_CONTAINMENT : MH_FUSE_NONE;
         q.shots = request.want_counts ? request.shots : 0;
-        q.want_statevector = request.want_statevector || (request.want_counts && !request.is_sole_region);
+        q.want_statevector = request.want_statevector ? 1 : 0;
         q.num_terms = static_cast<std::uint32_t>(o.x.size());
         q.x_masks = o.x.data();
         q.z_masks = o.z.data();
@@ -211,7 +287,7 @@ double HipDenseEngine::device_seconds(const EngineEstimateRequest& request) cons
         q.precision = MH_C128;
         q.fusion = request.fuse_gates ? MH_FUSE_CONTAINMENT : MH_FUSE_NONE;
         q.shots = request.want_counts ? request.shots : 0;
-        q.want_statevector = request.want_statevector || (request.want_counts && !request.is_sole_region);
+        q.want_statevector = request.want_statevector ? 1 : 0;
         q.num_terms = static_cast<std::uint32_t>(o.x.size());
         q.x_masks = o.x.data();
         q.z_masks = o.z.data();
@@ -235,9 +311,10 @@ EngineExecutionResult HipDenseEngine::execute(const EngineExecutionRequest& requ
     const Outputs o = outputs_of(request.want_expectation, request.observable, request.measure_targets);
     const std::uint64_t amplitudes = std::uint64_t{1} << request.num_qubits;
     const bool sole_counts = request.want_counts && request.is_sole_region;
-    // A non-sole region's sampler draws from the copied-back state, as the
-    // CUDA engine's does (the reference's CumulativeDistribution).
-    const bool want_host_state = request.want_statevector || (request.want_counts && !request.is_sole_region);
+    // A non-sole region's sampler keeps the final state on the device
+    // (HipRegionSampler) instead of a host copy.
+    const bool region_sampler = request.want_counts && !request.is_sole_region;
+    const bool want_host_state = request.want_statevector;
 
     mh_execution_request q{};
     q.circuit = &flat.view();
@@ -254,6 +331,7 @@ EngineExecutionResult HipDenseEngine::execute(const EngineExecutionRequest& requ
     q.x_masks = o.x.data();
     q.z_masks = o.z.data();
     q.want_statevector = want_host_state ? 1 : 0;
+    q.want_region_sampler = region_sampler ? 1 : 0;
     MonitorBridge bridge{request.monitor, nullptr};
     if (request.monitor != nullptr) {
         q.monitor = &MonitorBridge::call;
@@ -280,6 +358,10 @@ EngineExecutionResult HipDenseEngine::execute(const EngineExecutionRequest& requ
     const int st = mh_engine_execute(&q, &res);
     if (st == MH_ERR_ABANDONED && bridge.thrown) std::rethrow_exception(bridge.thrown);
     if (st != MH_OK) throw_status(st, "HipDenseEngine::execute");
+    std::unique_ptr<RegionCountsSampler> sampler;
+    if (res.region_sampler != nullptr)
+        sampler = std::make_unique<HipRegionSampler>(res.region_sampler, request.measure_targets,
+                                                     request.bitstring_width, request.shots);
     // The device's peak for this call, on the calling thread's ledger.
     { const common::MemoryLedger::Charge device(res.peak_device_bytes, common::MemoryLedger::Space::Device); }
 
@@ -296,8 +378,7 @@ EngineExecutionResult HipDenseEngine::execute(const EngineExecutionRequest& requ
             hist.emplace(bitstring_of(outcomes[i], request.bitstring_width), counts[i]);
         result.counts = std::move(hist);
     } else if (request.want_counts) {
-        result.counts_sampler = std::make_unique<engines::dense::CumulativeDistribution>(
-            host, request.measure_targets, request.bitstring_width);
+        result.counts_sampler = std::move(sampler);
     }
     if (request.want_statevector) {
         host_charge->release();

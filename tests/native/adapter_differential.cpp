@@ -334,6 +334,15 @@ void device_cases() {
         common::Rng ra(41), rb(41);
         for (int i = 0; same && i < 2000; ++i) same = a.counts_sampler->sample_one(ra) == b.counts_sampler->sample_one(rb);
         check(same, "non-sole region sampler draws");
+        // A caller that also draws from the stream between calls: the HIP
+        // sampler's look-ahead no longer matches its words, and every draw
+        // stays the reference's.
+        for (int i = 0; same && i < 64; ++i) {
+            (void)ra.next_word();
+            (void)rb.next_word();
+            same = a.counts_sampler->sample_one(ra) == b.counts_sampler->sample_one(rb);
+        }
+        check(same, "non-sole region sampler draws with other draws in between");
     }
 
     // Raw expectation terms within 1e-12, in term order.
