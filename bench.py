@@ -50,38 +50,47 @@ def host_cores() -> int:
 
 
 def cpu_baseline(qubits: int, layers: int, threads: int, gpu_passes: int, full_qubits: int,
-                 full_layers: int) -> dict:
+                 full_layers: int, second_qubits: int = 30, second_layers: int = 2) -> dict:
     """The reference CPU engine (oracle/_ref, DenseEngine::execute with the
-    reference's containment plan) on a bounded sample of the same workload:
-    the random circuit at `qubits` qubits and `layers` layers. `value` is in
-    the unit of the GPU figure: the GPU pass plan's algorithmic bytes for the
-    same sample circuit (gpu_passes x 2 x 16 x 2^qubits) over the CPU's
-    execute time, so value / cpu value is the speed-up on the same circuit.
-    The full workload's CPU time is extrapolated from the sample (state size
-    and layer count scale the time linearly; the sample's state is already far
-    beyond the host caches) and labelled as such."""
+    reference's containment plan) on bounded samples of the same workload: the
+    random circuit at `qubits` qubits and `layers` layers, then at
+    `second_qubits` qubits and `second_layers` layers. `value` is in the unit of
+    the GPU figure: the GPU pass plan's algorithmic bytes for the first sample
+    circuit (gpu_passes x 2 x 16 x 2^qubits) over the CPU's execute time, so
+    value / cpu value is the speed-up on the same circuit. Seconds per layer are
+    reported at both sizes; the full workload's CPU time is extrapolated from
+    them (the per-qubit growth measured between the two samples, applied from
+    the larger one) and labelled as such."""
     import oracle
     from menos_amd import workloads as W
-    c = W.random_circuit(qubits, layers, 2025)
-    if oracle.Reference.available():
-        _, _, _, secs = oracle.Reference().execute(c, fuse_gates=True, statevector=False, threads=threads)
-        kind = "reference"
-    else:
+
+    def run(n, depth):
+        c = W.random_circuit(n, depth, 2025)
+        if oracle.Reference.available():
+            _, _, _, secs = oracle.Reference().execute(c, fuse_gates=True, statevector=False, threads=threads)
+            return secs, "reference"
         t0 = time.perf_counter()
         oracle.Oracle().execute(c, fuse_gates=True, statevector=False, threads=threads)
-        secs = time.perf_counter() - t0
-        kind = "port"
+        return time.perf_counter() - t0, "port"
+
+    secs, kind = run(qubits, layers)
+    secs2, _ = run(second_qubits, second_layers)
     algo = gpu_passes * 2 * 16 * (1 << qubits)
     per_layer = secs / layers
-    full = per_layer * (1 << (full_qubits - qubits)) * full_layers
+    per_layer2 = secs2 / second_layers
+    growth = (per_layer2 / per_layer) ** (1.0 / (second_qubits - qubits))
+    full = per_layer2 * growth ** (full_qubits - second_qubits) * full_layers
     return {"value": round(algo / secs / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": kind,
             "sample": f"random circuit {qubits} qubits complex128, {layers} layers, reference containment plan, "
                       f"{secs:.2f} s on {threads} threads (nproc {host_cores()}); GB/s = the GPU pass plan's "
-                      f"{gpu_passes} passes x 2 x 16 x 2^{qubits} B over that time",
-            "seconds_per_layer": round(per_layer, 4),
+                      f"{gpu_passes} passes x 2 x 16 x 2^{qubits} B over that time; then {second_qubits} qubits, "
+                      f"{second_layers} layers, {secs2:.2f} s",
+            "seconds_per_layer": {str(qubits): round(per_layer, 4), str(second_qubits): round(per_layer2, 4)},
+            "growth_per_qubit": round(growth, 3),
             "extrapolated_full_seconds": round(full, 1),
-            "extrapolated_full": f"{full_qubits} qubits, {full_layers} layers: {full:.0f} s (extrapolated from "
-                                 f"the {qubits}-qubit sample, not measured)"}
+            "extrapolated_full": f"{full_qubits} qubits, {full_layers} layers: {full:.0f} s (extrapolated from the "
+                                 f"{qubits}- and {second_qubits}-qubit samples at x{growth:.2f} per qubit, "
+                                 "not measured)"}
 
 
 def batched(args, H, W) -> dict:

@@ -148,7 +148,8 @@ void EarlyReductions::after(const BatchLaunch& l, hipStream_t stream) {
     hipEvent_t done = nullptr;
     for (Bucket& b : buckets_) {
         if (b.queued || b.group != l.group || b.pass != l.pass) continue;
-        if (done == nullptr) {
+        // On the launch's own stream the order is already the stream's.
+        if (done == nullptr && stream != reduce_) {
             done = event();
             MH_HIP(hipEventRecord(done, stream));
             MH_HIP(hipStreamWaitEvent(reduce_, done, 0));

@@ -132,14 +132,12 @@ struct mh_batch_graph {
     mh::Pinned pinned_mats, pinned_terms;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    std::vector<hipEvent_t> events;
     uint32_t launches = 0;
     double plan_seconds = 0, compile_seconds = 0;
 
     ~mh_batch_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        for (hipEvent_t e : events) (void)hipEventDestroy(e);
     }
 };
 
@@ -173,13 +171,19 @@ void set_phases(std::vector<BatchItem>& items, const double* global_phases) {
     }
 }
 
+// Everything on the main stream. A graph captured over several streams (the
+// kernel streams forked off the main stream and joined back) kept the device
+// memory its kernels used referenced after the graph and its executable were
+// destroyed (measured on ROCm 7.2: hipFree returned success and the memory did
+// not come back), so resident batches were never released; captured on one
+// stream, the memory returns. One launch group at a time on one stream costs
+// the default schedule nothing (it has one launch group), and the reduction
+// buckets follow the launch that finishes their states.
 void capture(mh_batch_graph& g) {
     context();
-    const Streams& st = streams();
-    const hipStream_t main = st.main;
-    const uint32_t nstreams = std::max(1u, std::min(g.schedule.kernel_streams, Streams::kKernelStreams));
-    g.events.resize(2 * Streams::kKernelStreams);
-    for (hipEvent_t& e : g.events) MH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    const hipStream_t main = streams().main;
+    BatchSchedule one = g.schedule;
+    one.kernel_streams = 1;
     BatchDeviceTables dev;
     dev.bundles = g.bundles.as<uint64_t>();
     dev.ops = static_cast<const WOp*>(g.ops.get());
@@ -190,26 +194,11 @@ void capture(mh_batch_graph& g) {
     MH_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
     try {
         MH_HIP(hipMemcpyAsync(g.mats.get(), g.pinned_mats.get(), g.program.mats().size(), hipMemcpyHostToDevice, main));
-        // Fork the kernel streams off the main stream, run the program, join.
-        MH_HIP(hipEventRecord(g.events[0], main));
-        for (uint32_t k = 1; k < nstreams; ++k) MH_HIP(hipStreamWaitEvent(st.kernel(k), g.events[0], 0));
-        // The term reductions on a stream of their own, each bucket behind the
-        // launch that finishes its states (EarlyReductions).
-        const hipStream_t reduce = st.reduce;
-        std::vector<hipStream_t> used{main};
-        for (uint32_t k = 1; k < nstreams; ++k) used.push_back(st.kernel(k));
-        if (g.reductions) MH_HIP(hipStreamWaitEvent(reduce, g.events[0], 0));
         AfterLaunch after;
         if (g.reductions) after = [&](const BatchLaunch& l, hipStream_t s) { g.reductions->after(l, s); };
-        g.launches = issue_batch_program(g.program, g.items.data(), dev, g.schedule, 0, after);
-        if (g.reductions) g.reductions->flush(used);
-        for (uint32_t k = 1; k < nstreams; ++k) {
-            MH_HIP(hipEventRecord(g.events[Streams::kKernelStreams + k], st.kernel(k)));
-            MH_HIP(hipStreamWaitEvent(main, g.events[Streams::kKernelStreams + k], 0));
-        }
+        g.launches = issue_batch_program(g.program, g.items.data(), dev, one, 0, after);
         if (g.reductions) {
-            MH_HIP(hipEventRecord(g.events[Streams::kKernelStreams], reduce));
-            MH_HIP(hipStreamWaitEvent(main, g.events[Streams::kKernelStreams], 0));
+            g.reductions->flush({main});
             MH_HIP(hipMemcpyAsync(g.pinned_terms.get(), g.dterms.get(), g.dterms.bytes(), hipMemcpyDeviceToHost, main));
             g.launches += g.reductions->launches();
         }
@@ -332,7 +321,7 @@ int mh_batch_graph_create(const mh_batch_request* req, mh_batch_graph** out) {
         if (any_terms) {
             g->dterms = DeviceBuffer(uint64_t(g->total_terms) * 2 * sizeof(double));
             g->reductions = std::make_unique<EarlyReductions>(req->precision, g->program, exp_items,
-                                                              g->dterms.as<double>(), streams().reduce);
+                                                              g->dterms.as<double>(), streams().main);
             g->reductions->upload(streams().copy, false);
             MH_HIP(hipStreamSynchronize(streams().copy));
             g->pinned_terms = Pinned(g->dterms.bytes());
