@@ -323,6 +323,67 @@ typedef struct mh_cost_estimate {
 int mh_engine_estimate(const mh_execution_request* request, mh_cost_estimate* out);
 int mh_engine_execute(const mh_execution_request* request, mh_execution_result* result);
 
+/* ---- The host-offloaded tier ------------------------------------------------
+ * The dense algorithm over a state held in pinned host memory and run on the
+ * device through windows: the reference's Tier::HostOffloaded
+ * (include/menos/plan.h:88-98, measured there and not built), on the pass plan
+ * of its disk tier (src/engines/dense_tiered/pass_plan.h:12-102) and the shape
+ * of its chunked executor (chunked_executor.h:54-75). A state of n qubits is
+ * 2^(n-L) chunks of 2^L amplitudes (qubits below L local, the rest global); a
+ * device window of 2^W amplitudes brings 2^(W-L) chunks together, so a pass
+ * may hold any set of at most W - L global qubits. The passes follow the
+ * reference's greedy rule over the plan's blocks; each block runs on the
+ * windows with its operands remapped, by the in-memory engine's kernels. The
+ * statevector and counts are mh_engine_execute's for the same request bit for
+ * bit; raw terms agree with its terms to rounding (FP64 sums, in another
+ * order). The request's fields mean what they mean there; budget_bytes bounds
+ * the device bytes (two windows and the outputs' tables), and
+ * want_region_sampler is refused (no state stays on the device). */
+typedef struct mh_tier_pass {
+    uint64_t global_mask;   /* bit (g - chunk_qubits) for each global qubit g of the pass's set */
+    uint32_t first_block;   /* the plan's blocks [first_block, end_block) */
+    uint32_t end_block;
+} mh_tier_pass;
+
+/* The reference's tier seams (simulation_engine.h:189-190). */
+typedef struct mh_tier_request {
+    uint32_t chunk_qubits;    /* L; 0: the chunk with the fewest passes (choose_pass_plan, pass_plan.h:82-101),
+                                 at least W - 6 (at most 64 chunks to a window) */
+    uint32_t window_qubits;   /* W; 0: the largest the budget holds (no budget: min(n, 28)) */
+    mh_tier_pass* passes;     /* optional: receives up to pass_capacity of the plan's passes */
+    uint32_t pass_capacity;
+    uint32_t reserved0;
+} mh_tier_request;
+
+/* The reference's TierCost (include/menos/plan.h:177-196) and the device side. */
+typedef struct mh_tier_cost {
+    int32_t feasible;
+    char infeasible_reason[256];
+    uint64_t passes;              /* gate passes over the whole state */
+    uint32_t chunk_qubits;
+    uint32_t window_qubits;
+    uint64_t link_bytes;          /* host<->device bytes: each pass's write-back and read, the outputs' reads */
+    uint64_t host_state_bytes;    /* the pinned state, 2^n amplitudes (the disk tier's file_bytes) */
+    uint64_t link_write_bytes;    /* the device->host part of link_bytes */
+    uint64_t global_blocks;       /* blocks with an operand at or above the chunk */
+    uint32_t max_global_operands;
+    uint32_t blocks;
+    uint64_t device_bytes;        /* two windows, the sampler's tables, the expectation partials */
+    double seconds;               /* the calibrated time model (mh_calibration's h2d/d2h and pass rates) */
+    int32_t fusion_used;
+    uint32_t fusion_width_used;
+} mh_tier_cost;
+
+/* Host work only; never fails on an infeasible request (feasible = 0 and a
+ * reason), as DiskDenseEngine::estimate (disk_dense_engine.cpp:307-341). */
+int mh_tier_estimate(const mh_execution_request* request, const mh_tier_request* tier, mh_tier_cost* out);
+/* Runs the plan mh_tier_estimate reports (written to `plan` when not null);
+ * num_passes counts tier passes, pass_ms their wall times, actual_bytes the
+ * windows; the monitor is called at the end of each pass (chunked_executor.cpp:
+ * 216-221), with the passes completed. */
+int mh_tier_execute(const mh_execution_request* request, const mh_tier_request* tier, mh_execution_result* result,
+                    mh_tier_cost* plan);
+
 /* Basis indices for `count` uniforms in [0, 1), in any order: index i is the
  * first basis state whose running sum fl(c + |a|^2) in index order exceeds
  * uniforms[i] (the last one if none), as CumulativeDistribution::sample_one

@@ -61,13 +61,6 @@ mh_calibration g_calibration = {
 
 double word_to_uniform(uint64_t w) { return static_cast<double>(w >> 11) * (1.0 / 9007199254740992.0); }
 
-struct Plan {
-    FusionPlan fusion;
-    std::vector<Block> blocks;
-    int32_t fusion_used = MH_FUSE_CONTAINMENT;
-    uint32_t width_used = 0;
-};
-
 uint32_t window_for(const ExecutionRequest& r) {
     return r.window_qubits == 0 ? window_default(r.precision) : r.window_qubits;
 }
@@ -96,6 +89,30 @@ double pass_seconds(uint32_t n, int32_t precision, const std::vector<Block>& blo
         seconds += std::max(2.0 * sb / c.hbm_bytes_per_second, ops / rate);
     }
     return seconds;
+}
+
+}  // namespace
+
+const mh_calibration& calibration() { return g_calibration; }
+
+std::vector<double> sorted_uniforms(uint64_t seed, uint64_t shots) {
+    std::mt19937_64 rng(seed);
+    std::vector<double> u(shots);
+    for (auto& x : u) x = word_to_uniform(rng());
+    std::sort(u.begin(), u.end());
+    return u;
+}
+
+std::map<uint64_t, uint64_t> fold_counts(const std::vector<uint64_t>& basis,
+                                         const std::vector<std::pair<uint32_t, uint32_t>>& targets) {
+    std::map<uint64_t, uint64_t> counts;
+    for (uint64_t b : basis) {
+        uint64_t key = 0;
+        for (const auto& [q, c] : targets)
+            if ((b >> q) & 1ULL) key |= uint64_t{1} << c;
+        counts[key] += 1;
+    }
+    return counts;
 }
 
 Plan make_plan(const ExecutionRequest& r) {
@@ -134,8 +151,6 @@ Plan make_plan(const ExecutionRequest& r) {
     }
     return p;
 }
-
-}  // namespace
 
 CostEstimate HipDenseEngine::estimate(const ExecutionRequest& r) const noexcept {
     CostEstimate e;
@@ -186,20 +201,10 @@ CostEstimate HipDenseEngine::estimate(const ExecutionRequest& r) const noexcept 
 
 std::map<uint64_t, uint64_t> sample_counts(const void* state, uint32_t n, int32_t precision, uint64_t shots,
                                            uint64_t seed, const std::vector<std::pair<uint32_t, uint32_t>>& targets) {
-    std::mt19937_64 rng(seed);
-    std::vector<double> u(shots);
-    for (auto& x : u) x = word_to_uniform(rng());
-    std::sort(u.begin(), u.end());
+    const std::vector<double> u = sorted_uniforms(seed, shots);
     std::vector<uint64_t> basis(shots);
     sample_sorted(state, n, precision, u.data(), shots, basis.data());
-    std::map<uint64_t, uint64_t> counts;
-    for (uint64_t b : basis) {
-        uint64_t key = 0;
-        for (const auto& [q, c] : targets)
-            if ((b >> q) & 1ULL) key |= uint64_t{1} << c;
-        counts[key] += 1;
-    }
-    return counts;
+    return fold_counts(basis, targets);
 }
 
 ExecutionResult HipDenseEngine::execute(const ExecutionRequest& r) const {
@@ -294,9 +299,7 @@ ExecutionResult HipDenseEngine::execute(const ExecutionRequest& r) const {
 
 }  // namespace mh
 
-using namespace mh;
-
-namespace {
+namespace mh {
 
 ExecutionRequest to_request(const mh_execution_request* q) {
     if (q == nullptr || q->circuit == nullptr) fail(MH_ERR_INVALID, "null request or circuit");
@@ -322,7 +325,43 @@ ExecutionRequest to_request(const mh_execution_request* q) {
     return r;
 }
 
-}  // namespace
+void write_result(ExecutionResult& res, mh_execution_result* out) {
+    out->region_sampler = res.region_sampler;
+    res.region_sampler = nullptr;
+    if (res.statevector && out->statevector) std::memcpy(out->statevector, res.statevector->data(), res.statevector->size());
+    out->num_outcomes = 0;
+    if (res.counts && out->outcomes && out->counts) {
+        uint64_t at = 0;
+        for (const auto& [k, v] : *res.counts) {
+            out->outcomes[at] = k;
+            out->counts[at] = v;
+            ++at;
+        }
+        out->num_outcomes = at;
+    }
+    if (res.expectation_terms && out->expectation_terms)
+        std::memcpy(out->expectation_terms, res.expectation_terms->data(), res.expectation_terms->size() * sizeof(cplx));
+    if (out->pass_ms) {
+        const std::size_t m = std::min<std::size_t>(out->pass_ms_capacity, res.pass_ms.size());
+        std::copy(res.pass_ms.begin(), res.pass_ms.begin() + static_cast<std::ptrdiff_t>(m), out->pass_ms);
+    }
+    out->num_blocks = res.num_blocks;
+    out->num_passes = res.num_passes;
+    out->fusion_used = res.fusion_used;
+    out->fusion_width_used = res.width_used;
+    out->fused_blocks = res.fused_blocks;
+    out->gates_absorbed = res.gates_absorbed;
+    out->actual_bytes = res.actual_bytes;
+    out->peak_device_bytes = res.peak_device_bytes;
+    out->execute_seconds = res.execute_seconds;
+    out->gate_seconds = res.gate_seconds;
+    out->sample_seconds = res.sample_seconds;
+    out->expectation_seconds = res.expectation_seconds;
+}
+
+}  // namespace mh
+
+using namespace mh;
 
 extern "C" {
 
@@ -359,36 +398,7 @@ int mh_engine_execute(const mh_execution_request* request, mh_execution_result* 
         out->region_sampler = nullptr;
         const ExecutionRequest r = to_request(request);
         ExecutionResult res = HipDenseEngine{}.execute(r);
-        out->region_sampler = res.region_sampler;
-        if (res.statevector && out->statevector) std::memcpy(out->statevector, res.statevector->data(), res.statevector->size());
-        out->num_outcomes = 0;
-        if (res.counts && out->outcomes && out->counts) {
-            uint64_t at = 0;
-            for (const auto& [k, v] : *res.counts) {
-                out->outcomes[at] = k;
-                out->counts[at] = v;
-                ++at;
-            }
-            out->num_outcomes = at;
-        }
-        if (res.expectation_terms && out->expectation_terms)
-            std::memcpy(out->expectation_terms, res.expectation_terms->data(), res.expectation_terms->size() * sizeof(cplx));
-        if (out->pass_ms) {
-            const std::size_t m = std::min<std::size_t>(out->pass_ms_capacity, res.pass_ms.size());
-            std::copy(res.pass_ms.begin(), res.pass_ms.begin() + static_cast<std::ptrdiff_t>(m), out->pass_ms);
-        }
-        out->num_blocks = res.num_blocks;
-        out->num_passes = res.num_passes;
-        out->fusion_used = res.fusion_used;
-        out->fusion_width_used = res.width_used;
-        out->fused_blocks = res.fused_blocks;
-        out->gates_absorbed = res.gates_absorbed;
-        out->actual_bytes = res.actual_bytes;
-        out->peak_device_bytes = res.peak_device_bytes;
-        out->execute_seconds = res.execute_seconds;
-        out->gate_seconds = res.gate_seconds;
-        out->sample_seconds = res.sample_seconds;
-        out->expectation_seconds = res.expectation_seconds;
+        write_result(res, out);
     });
 }
 

@@ -111,6 +111,32 @@ struct CostEstimate {
 std::map<uint64_t, uint64_t> sample_counts(const void* state, uint32_t n, int32_t precision, uint64_t shots,
                                            uint64_t seed, const std::vector<std::pair<uint32_t, uint32_t>>& targets);
 
+// The blocks a request runs: its fusion rule's, or the autotuner's choice
+// (containment, then widths 2..6, priced by the time model).
+struct Plan {
+    FusionPlan fusion;
+    std::vector<Block> blocks;
+    int32_t fusion_used = MH_FUSE_CONTAINMENT;
+    uint32_t width_used = 0;
+};
+Plan make_plan(const ExecutionRequest& request);
+
+// The installed time-model profile (mh_set_calibration, mh_calibrate).
+const mh_calibration& calibration();
+
+// The sampler's uniforms: mt19937_64(seed) words, 53 bits each, sorted
+// (reference src/engines/dense/sample.cpp:24-75).
+std::vector<double> sorted_uniforms(uint64_t seed, uint64_t shots);
+// Basis indices folded into counts keyed by the clbit string (bit q of an
+// index lands on clbit c for every (q, c) target).
+std::map<uint64_t, uint64_t> fold_counts(const std::vector<uint64_t>& basis,
+                                         const std::vector<std::pair<uint32_t, uint32_t>>& targets);
+
+// The C request as the engine's, and the engine's result into the caller's
+// buffers (ownership of a region sampler passes to the caller).
+ExecutionRequest to_request(const mh_execution_request* request);
+void write_result(ExecutionResult& result, mh_execution_result* out);
+
 class HipDenseEngine {
 public:
     const char* name() const { return "dense_hip"; }

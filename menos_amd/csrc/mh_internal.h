@@ -208,6 +208,31 @@ void sample_sorted(const void* state, uint32_t n, int32_t precision, const doubl
 // (mh_region_sampler_*); synchronises.
 mh_region_sampler* make_region_sampler(DeviceBuffer&& state, uint32_t n, int32_t precision);
 
+// sample_sorted over a state of 2^n amplitudes that reaches the device in
+// consecutive chunks of 2^chunk_log (index order), with the same indices bit
+// for bit: each chunk's tables start from the exact running value the chunk
+// before it ended on, and a uniform is drawn in the chunk whose range of
+// running values holds it (one at or above the total takes the last index).
+// Everything is queued on `stream`; a chunk's memory may be reused once the
+// work queued by its add() is done.
+class ChunkedSampler {
+public:
+    ChunkedSampler(uint32_t n, uint32_t chunk_log, int32_t precision, const double* sorted_u, uint64_t shots,
+                   hipStream_t stream);
+    ~ChunkedSampler();
+    ChunkedSampler(const ChunkedSampler&) = delete;
+    ChunkedSampler& operator=(const ChunkedSampler&) = delete;
+    // The next chunk, at `chunk` (device memory, ready in stream order).
+    void add(const void* chunk);
+    // After the last chunk: waits, and returns each uniform's basis index.
+    std::vector<uint64_t> basis();
+    uint64_t device_bytes() const;
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
+
 // ---- host IR (host_ir.cpp)
 struct CircuitView {
     const mh_circuit* c;

@@ -138,7 +138,12 @@ struct SampleArgs {
     Xfer* gxfer;          // [groups]
     double* cstart;       // [blocks + 1] exact c before each block
     uint32_t* jumped;     // [groups]
+    // The exact running value before the first element (device; null: 0),
+    // for a state that arrives in consecutive chunks.
+    const double* carry;
 };
+
+__device__ __forceinline__ double carry_of(const SampleArgs& a) { return a.carry ? *a.carry : 0.0; }
 
 template <typename V>
 __global__ void __launch_bounds__(kThreads) block_sums_kernel(SampleArgs a) {
@@ -167,7 +172,7 @@ __global__ void __launch_bounds__(1024) approx_prefix_kernel(SampleArgs a) {
     part[threadIdx.x] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double run = 0.0;
+        double run = carry_of(a);
         for (int t = 0; t < 1024; ++t) {
             const double v = part[t];
             part[t] = run;
@@ -248,10 +253,10 @@ __global__ void __launch_bounds__(kThreads) chain_kernel(SampleArgs a) {
     const uint32_t subs = static_cast<uint32_t>(a.block / kSub) == 0 ? 1 : static_cast<uint32_t>(a.block / kSub);
     const uint32_t sub_len = static_cast<uint32_t>(a.block / subs);
     if (threadIdx.x == 0) {
-        c_sh = 0.0;
+        c_sh = carry_of(a);
         b_sh = 0;
         mode_sh = kModeGroup;
-        a.cstart[0] = 0.0;
+        a.cstart[0] = c_sh;
     }
     __syncthreads();
     while (true) {
@@ -406,6 +411,45 @@ __global__ void __launch_bounds__(kThreads) shots_kernel(SampleArgs a, const dou
     out[t] = hit;
 }
 
+// The draws of one chunk of a state that arrives in chunks (index order): a
+// uniform is this chunk's when the running value before the chunk is at most
+// it and the value after the chunk exceeds it (those ranges partition [0,
+// total)); its index is the chunk's first index plus the hit inside it, found
+// as shots_kernel finds it. Other uniforms are left alone.
+template <typename V>
+__global__ void __launch_bounds__(kThreads) chunk_shots_kernel(SampleArgs a, const double* u, uint64_t shots,
+                                                              uint64_t* out, uint64_t first_index) {
+    const uint64_t t = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (t >= shots) return;
+    const double x = u[t];
+    if (!(x >= a.cstart[0] && x < a.cstart[a.blocks])) return;
+    uint32_t lo = 0, hi = a.blocks;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (x < a.cstart[mid + 1])
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    const V* st = static_cast<const V*>(a.state);
+    double c = a.cstart[lo];
+    const uint64_t i0 = uint64_t(lo) * a.block;
+    uint64_t hit = i0 + a.block - 1;
+    for (uint64_t i = 0; i < a.block; ++i) {
+        c = c + prob(st, i0 + i);
+        if (x < c) {
+            hit = i0 + i;
+            break;
+        }
+    }
+    out[t] = first_index + hit;
+}
+
+__global__ void __launch_bounds__(kThreads) fill_index_kernel(uint64_t* out, uint64_t count, uint64_t value) {
+    const uint64_t t = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (t < count) out[t] = value;
+}
+
 // The first half of the sampler over one state: the exact running value at
 // every block start (block sums, predicted binades, composed transfers, the
 // chain, the starts inside jumped groups), in device tables kept for draws.
@@ -416,8 +460,8 @@ struct SamplerTables {
 
 uint64_t r16(uint64_t bytes) { return (bytes + 15) / 16 * 16; }
 
-template <typename V>
-void prepare_tables(SamplerTables& t, const void* state, uint32_t n) {
+// Allocates the tables of a state of 2^n amplitudes at `state`.
+void layout_tables(SamplerTables& t, const void* state, uint32_t n) {
     context();
     SampleArgs& a = t.a;
     a.state = state;
@@ -440,18 +484,30 @@ void prepare_tables(SamplerTables& t, const void* state, uint32_t n) {
     a.approx = reinterpret_cast<double*>(take((nb + 1) * sizeof(double)));
     a.cstart = reinterpret_cast<double*>(take((nb + 1) * sizeof(double)));
     a.jumped = reinterpret_cast<uint32_t*>(take(uint64_t(a.groups) * sizeof(uint32_t)));
-    hipLaunchKernelGGL(block_sums_kernel<V>, dim3(a.blocks), dim3(kThreads), 0, streams().main, a);
+    a.carry = nullptr;
+}
+
+// Queues the table kernels for the state and carry in `a` on `stream`.
+template <typename V>
+void launch_tables(const SampleArgs& a, hipStream_t stream) {
+    hipLaunchKernelGGL(block_sums_kernel<V>, dim3(a.blocks), dim3(kThreads), 0, stream, a);
     MH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(approx_prefix_kernel, dim3(1), dim3(1024), 0, streams().main, a);
+    hipLaunchKernelGGL(approx_prefix_kernel, dim3(1), dim3(1024), 0, stream, a);
     MH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(block_xfer_kernel<V>, dim3(a.blocks), dim3(kThreads), 0, streams().main, a);
+    hipLaunchKernelGGL(block_xfer_kernel<V>, dim3(a.blocks), dim3(kThreads), 0, stream, a);
     MH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(group_xfer_kernel, dim3(a.groups), dim3(kThreads), 0, streams().main, a);
+    hipLaunchKernelGGL(group_xfer_kernel, dim3(a.groups), dim3(kThreads), 0, stream, a);
     MH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(chain_kernel<V>, dim3(1), dim3(kThreads), 0, streams().main, a);
+    hipLaunchKernelGGL(chain_kernel<V>, dim3(1), dim3(kThreads), 0, stream, a);
     MH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(expand_kernel, dim3(a.groups), dim3(kThreads), 0, streams().main, a);
+    hipLaunchKernelGGL(expand_kernel, dim3(a.groups), dim3(kThreads), 0, stream, a);
     MH_HIP(hipGetLastError());
+}
+
+template <typename V>
+void prepare_tables(SamplerTables& t, const void* state, uint32_t n) {
+    layout_tables(t, state, n);
+    launch_tables<V>(t.a, streams().main);
 }
 
 // The second half: each uniform's basis index (any order), waited for.
@@ -490,6 +546,93 @@ void sample_sorted(const void* state, uint32_t n, int32_t precision, const doubl
         run_sampler<double2>(state, n, sorted_u, shots, basis_out);
     else
         run_sampler<float2>(state, n, sorted_u, shots, basis_out);
+}
+
+struct ChunkedSampler::Impl {
+    uint32_t n = 0, chunk_log = 0;
+    int32_t precision = MH_C128;
+    uint64_t shots = 0, next = 0;
+    hipStream_t stream = nullptr;
+    SamplerTables tables[2];
+    DeviceBuffer io;  // uniforms, basis indices, the zero carry of chunk 0
+    double* du = nullptr;
+    uint64_t* dout = nullptr;
+    double* zero = nullptr;
+};
+
+ChunkedSampler::ChunkedSampler(uint32_t n, uint32_t chunk_log, int32_t precision, const double* sorted_u,
+                               uint64_t shots, hipStream_t stream)
+    : impl_(std::make_unique<Impl>()) {
+    amp_bytes(precision);
+    if (n > 40 || chunk_log > n) fail(MH_ERR_INVALID, "chunked sampler: chunk above the state or above 40 qubits");
+    for (uint64_t i = 1; i < shots; ++i)
+        if (!(sorted_u[i - 1] <= sorted_u[i])) fail(MH_ERR_INVALID, "sample: uniforms are not sorted");
+    Impl& m = *impl_;
+    m.n = n;
+    m.chunk_log = chunk_log;
+    m.precision = precision;
+    m.shots = shots;
+    m.stream = stream;
+    for (SamplerTables& t : m.tables) layout_tables(t, nullptr, chunk_log);
+    m.io = DeviceBuffer(r16(std::max<uint64_t>(shots, 1) * sizeof(double)) +
+                        r16(std::max<uint64_t>(shots, 1) * sizeof(uint64_t)) + 16);
+    m.du = m.io.as<double>();
+    m.dout = reinterpret_cast<uint64_t*>(m.io.as<char>() + r16(std::max<uint64_t>(shots, 1) * sizeof(double)));
+    m.zero = reinterpret_cast<double*>(reinterpret_cast<char*>(m.dout) + r16(std::max<uint64_t>(shots, 1) * sizeof(uint64_t)));
+    MH_HIP(hipMemsetAsync(m.zero, 0, sizeof(double), stream));
+    if (shots > 0) {
+        MH_HIP(hipMemcpyAsync(m.du, sorted_u, shots * sizeof(double), hipMemcpyHostToDevice, stream));
+        // A uniform at or above the total goes to the last index.
+        hipLaunchKernelGGL(fill_index_kernel, dim3(static_cast<uint32_t>((shots + kThreads - 1) / kThreads)),
+                           dim3(kThreads), 0, stream, m.dout, shots, (uint64_t{1} << n) - 1);
+        MH_HIP(hipGetLastError());
+        // The uniforms are the caller's memory: the copy is complete before it returns.
+        MH_HIP(hipStreamSynchronize(stream));
+    }
+}
+
+ChunkedSampler::~ChunkedSampler() {
+    if (impl_) (void)hipStreamSynchronize(impl_->stream);
+}
+
+uint64_t ChunkedSampler::device_bytes() const {
+    return impl_->tables[0].buf.bytes() + impl_->tables[1].buf.bytes() + impl_->io.bytes();
+}
+
+void ChunkedSampler::add(const void* chunk) {
+    Impl& m = *impl_;
+    const uint64_t c = m.next;
+    if (c >= (uint64_t{1} << (m.n - m.chunk_log))) fail(MH_ERR_INTERNAL, "chunked sampler: more chunks than the state has");
+    m.next = c + 1;
+    SampleArgs a = m.tables[c % 2].a;
+    a.state = chunk;
+    const SampleArgs& before = m.tables[(c + 1) % 2].a;
+    a.carry = c == 0 ? m.zero : before.cstart + before.blocks;
+    // The tables of chunk c + 2 take this slot: the stream orders them after
+    // this chunk's draws and after chunk c + 1 has read the carry.
+    m.tables[c % 2].a = a;
+    if (m.precision == MH_C128)
+        launch_tables<double2>(a, m.stream);
+    else
+        launch_tables<float2>(a, m.stream);
+    if (m.shots == 0) return;
+    const uint32_t grid = static_cast<uint32_t>((m.shots + kThreads - 1) / kThreads);
+    if (m.precision == MH_C128)
+        hipLaunchKernelGGL(chunk_shots_kernel<double2>, dim3(grid), dim3(kThreads), 0, m.stream, a,
+                           static_cast<const double*>(m.du), m.shots, m.dout, c << m.chunk_log);
+    else
+        hipLaunchKernelGGL(chunk_shots_kernel<float2>, dim3(grid), dim3(kThreads), 0, m.stream, a,
+                           static_cast<const double*>(m.du), m.shots, m.dout, c << m.chunk_log);
+    MH_HIP(hipGetLastError());
+}
+
+std::vector<uint64_t> ChunkedSampler::basis() {
+    Impl& m = *impl_;
+    if (m.next != (uint64_t{1} << (m.n - m.chunk_log))) fail(MH_ERR_INTERNAL, "chunked sampler: chunks missing");
+    std::vector<uint64_t> out(m.shots);
+    if (m.shots > 0) MH_HIP(hipMemcpyAsync(out.data(), m.dout, m.shots * sizeof(uint64_t), hipMemcpyDeviceToHost, m.stream));
+    MH_HIP(hipStreamSynchronize(m.stream));
+    return out;
 }
 
 }  // namespace mh

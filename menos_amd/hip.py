@@ -35,6 +35,7 @@ EXPORTS = (
     "mh_plan_passes", "mh_gather", "mh_batch_graph_create", "mh_batch_graph_update", "mh_batch_graph_run",
     "mh_batch_graph_destroy", "mh_batch_graph_set_params", "mh_batch_graph_param_count",
     "mh_region_sampler_draw", "mh_region_sampler_info", "mh_region_sampler_destroy",
+    "mh_tier_estimate", "mh_tier_execute",
 )
 JOB_ESTIMATOR, JOB_SAMPLER = 0, 1
 
@@ -390,6 +391,106 @@ class HipDenseEngine:
                       expectation_seconds=res.expectation_seconds, fusion_used=res.fusion_used,
                       fusion_width_used=res.fusion_width_used,
                       region_sampler=RegionSampler(res.region_sampler) if res.region_sampler else None)
+
+
+class CTierPass(ctypes.Structure):
+    _fields_ = [("global_mask", ctypes.c_uint64), ("first_block", ctypes.c_uint32), ("end_block", ctypes.c_uint32)]
+
+
+class CTierRequest(ctypes.Structure):
+    _fields_ = [("chunk_qubits", ctypes.c_uint32), ("window_qubits", ctypes.c_uint32),
+                ("passes", ctypes.POINTER(CTierPass)), ("pass_capacity", ctypes.c_uint32),
+                ("reserved0", ctypes.c_uint32)]
+
+
+class CTierCost(ctypes.Structure):
+    _fields_ = [("feasible", ctypes.c_int32), ("infeasible_reason", ctypes.c_char * 256),
+                ("passes", ctypes.c_uint64), ("chunk_qubits", ctypes.c_uint32), ("window_qubits", ctypes.c_uint32),
+                ("link_bytes", ctypes.c_uint64), ("host_state_bytes", ctypes.c_uint64),
+                ("link_write_bytes", ctypes.c_uint64), ("global_blocks", ctypes.c_uint64),
+                ("max_global_operands", ctypes.c_uint32), ("blocks", ctypes.c_uint32),
+                ("device_bytes", ctypes.c_uint64), ("seconds", ctypes.c_double), ("fusion_used", ctypes.c_int32),
+                ("fusion_width_used", ctypes.c_uint32)]
+
+
+class HipHostTierEngine:
+    """The {Dense, Hip, HostOffloaded} engine (mh_tier_*): the state in pinned
+    host memory, the passes run on the device through two windows. Takes
+    HipDenseEngine's arguments, plus the tier seams ``tier_chunk_qubits`` and
+    ``tier_window_qubits`` (0: chosen); ``plan`` holds the pass plan."""
+
+    name = "dense_hip_host_offloaded"
+
+    @staticmethod
+    def _tier(tier_chunk_qubits, tier_window_qubits, capacity):
+        passes = (CTierPass * max(capacity, 1))()
+        return CTierRequest(tier_chunk_qubits, tier_window_qubits, passes, capacity), passes
+
+    @staticmethod
+    def _plan(cost, passes):
+        count = min(int(cost.passes), len(passes))
+        return [(int(passes[i].global_mask), int(passes[i].first_block), int(passes[i].end_block))
+                for i in range(count)]
+
+    def estimate(self, circuit: W.Circuit, *, tier_chunk_qubits=0, tier_window_qubits=0, precision=C128,
+                 fusion=FUSE_CONTAINMENT, fusion_width=0, shots=0, seed=0, targets=None, width=None, terms=(),
+                 budget_bytes=0, statevector=False, max_passes=4096):
+        keep = _CircuitKeep(circuit)
+        req, hold = HipDenseEngine._request(circuit, keep, precision=precision, fusion=fusion,
+                                            fusion_width=fusion_width, window_qubits=0, shots=shots, seed=seed,
+                                            targets=targets, width=width, terms=terms, budget_bytes=budget_bytes,
+                                            statevector=statevector, record_pass_times=False)
+        tier, passes = self._tier(tier_chunk_qubits, tier_window_qubits, max_passes)
+        cost = CTierCost()
+        check(lib().mh_tier_estimate(ctypes.byref(req), ctypes.byref(tier), ctypes.byref(cost)))
+        cost.plan = self._plan(cost, passes) if cost.feasible else []
+        return cost
+
+    def execute(self, circuit: W.Circuit, *, tier_chunk_qubits=0, tier_window_qubits=0, precision=C128,
+                fusion=FUSE_CONTAINMENT, fusion_width=0, window_qubits=0, shots=0, seed=0, targets=None,
+                width=None, terms=(), budget_bytes=0, statevector=True, record_pass_times=False,
+                max_passes=4096, max_pass_cost=0, monitor=None) -> Result:
+        """``monitor(passes_completed)`` runs at the end of every tier pass and
+        abandons the run by raising, as in HipDenseEngine.execute."""
+        keep = _CircuitKeep(circuit)
+        req, hold = HipDenseEngine._request(circuit, keep, precision=precision, fusion=fusion,
+                                            fusion_width=fusion_width, window_qubits=window_qubits, shots=shots,
+                                            seed=seed, targets=targets, width=width, terms=terms,
+                                            budget_bytes=budget_bytes, statevector=statevector,
+                                            record_pass_times=record_pass_times, max_pass_cost=max_pass_cost)
+        raised = []
+        if monitor is not None:
+            def _boundary(_data, crossed):
+                try:
+                    monitor(int(crossed))
+                    return 0
+                except BaseException as e:  # handed back to the caller below
+                    raised.append(e)
+                    return 1
+            req.monitor = MONITOR_FN(_boundary)
+        tier, passes = self._tier(tier_chunk_qubits, tier_window_qubits, max_passes)
+        sv = np.zeros(1 << circuit.num_qubits, amp_dtype(precision)) if statevector else None
+        outc = np.zeros(max(shots, 1), np.uint64)
+        cnt = np.zeros(max(shots, 1), np.uint64)
+        ex = np.zeros(2 * max(len(terms), 1), np.float64)
+        pms = np.zeros(max_passes, np.float32)
+        res = CResult(sv.ctypes.data_as(ctypes.c_void_p) if sv is not None else None,
+                      outc.ctypes.data_as(_u64p), cnt.ctypes.data_as(_u64p), 0, ex.ctypes.data_as(_f64p),
+                      pms.ctypes.data_as(_f32p), max_passes)
+        cost = CTierCost()
+        status = lib().mh_tier_execute(ctypes.byref(req), ctypes.byref(tier), ctypes.byref(res), ctypes.byref(cost))
+        if status == MH_ERR_ABANDONED and raised:
+            raise raised[0]
+        check(status)
+        counts = {int(outc[i]): int(cnt[i]) for i in range(res.num_outcomes)} if shots else None
+        return Result(statevector=sv, counts=counts,
+                      expectation=ex.view(np.complex128)[:len(terms)].copy() if terms else None,
+                      pass_ms=pms[:min(res.num_passes, max_passes)] if record_pass_times else None,
+                      num_blocks=res.num_blocks, num_passes=res.num_passes, actual_bytes=res.actual_bytes,
+                      peak_device_bytes=res.peak_device_bytes, execute_seconds=res.execute_seconds,
+                      gate_seconds=res.gate_seconds, sample_seconds=res.sample_seconds,
+                      expectation_seconds=res.expectation_seconds, fusion_used=res.fusion_used,
+                      fusion_width_used=res.fusion_width_used, cost=cost, plan=self._plan(cost, passes))
 
 
 class RegionSampler:

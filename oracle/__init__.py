@@ -260,6 +260,29 @@ class Reference(_Api):
         fn(state.ctypes.data_as(_f64p), ctypes.c_uint32(n), m.ctypes.data_as(_f64p), _ptr(ops, _u32p),
            ctypes.c_uint32(len(ops)), ctypes.c_uint32(threads))
 
+    def tier_pass_plan(self, circuit, chunk_qubits, window_qubits, fuse=False, choose=False, capacity=4096):
+        """The reference disk tier's pass plan (build_pass_plan, or
+        choose_pass_plan with choose): dict(chunk, passes=[(mask, first, end)],
+        global_gates, max_global, too_small), or None when no chunk fits."""
+        cc, keep = c_circuit(circuit)
+        masks = np.zeros(capacity, np.uint64)
+        first = np.zeros(capacity, np.uint32)
+        end = np.zeros(capacity, np.uint32)
+        chunk, gg, mg, small = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_int32()
+        fn = self.lib.ref_tier_pass_plan
+        fn.restype = ctypes.c_uint32
+        count = fn(ctypes.byref(cc), ctypes.c_uint32(chunk_qubits), ctypes.c_uint32(window_qubits),
+                   ctypes.c_int32(int(fuse)), ctypes.c_int32(int(choose)), _ptr(masks, _u64p), _ptr(first, _u32p),
+                   _ptr(end, _u32p), ctypes.c_uint32(capacity), ctypes.byref(chunk), ctypes.byref(gg),
+                   ctypes.byref(mg), ctypes.byref(small))
+        del keep
+        if count == 0xFFFFFFFF:
+            return None
+        m = min(count, capacity)
+        return dict(chunk=chunk.value, count=count,
+                    passes=[(int(masks[i]), int(first[i]), int(end[i])) for i in range(m)],
+                    global_gates=gg.value, max_global=mg.value, too_small=bool(small.value))
+
     def run_plan(self, circuit, plan: Plan, threads=8):
         cc, keep = c_circuit(circuit)
         cp = plan.c()

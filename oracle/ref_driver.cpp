@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <optional>
 #include <vector>
 
 #include "common/memory_ledger.h"
@@ -21,6 +22,7 @@
 #include "engines/dense/dense_engine.h"
 #include "engines/dense/fused_block.h"
 #include "engines/dense/sample.h"
+#include "engines/dense_tiered/pass_plan.h"
 #include "ir/fuse_gates.h"
 #include "ir/gate_table.h"
 #include "kernels/dense_kernels.h"
@@ -101,6 +103,39 @@ uint32_t ref_plan_fusion(const orc_circuit* circuit, uint32_t* order, uint32_t* 
     *fused_blocks = p.fused_blocks;
     *gates_absorbed = p.gates_absorbed;
     return static_cast<uint32_t>(p.block_count());
+}
+
+// The disk tier's pass plan (engines/dense_tiered/pass_plan.h): over the
+// operations, or with fuse over ir::plan_fusion's blocks; chunk_qubits 0 with
+// choose set asks choose_pass_plan for the chunk. Writes up to `capacity`
+// passes; returns the pass count, or UINT32_MAX when no plan fits.
+uint32_t ref_tier_pass_plan(const orc_circuit* circuit, uint32_t chunk_qubits, uint32_t window_qubits, int32_t fuse,
+                            int32_t choose, uint64_t* masks, uint32_t* first, uint32_t* end, uint32_t capacity,
+                            uint32_t* chunk_out, uint64_t* global_gates, uint32_t* max_global, int32_t* too_small) {
+    const std::vector<menos::Operation> ops = operations_of(*circuit);
+    std::optional<menos::ir::FusionPlan> fusion;
+    if (fuse) fusion.emplace(menos::ir::plan_fusion(ops));
+    const menos::ir::FusionPlan* f = fusion ? &*fusion : nullptr;
+    menos::engines::dense_tiered::PassPlan plan;
+    if (choose) {
+        const auto chosen =
+            menos::engines::dense_tiered::choose_pass_plan(ops, circuit->num_qubits, window_qubits, false, f);
+        if (!chosen) return UINT32_MAX;
+        plan = *chosen;
+    } else {
+        plan = menos::engines::dense_tiered::build_pass_plan(ops, circuit->num_qubits, chunk_qubits, window_qubits, f);
+    }
+    *chunk_out = plan.chunk_qubits;
+    *global_gates = plan.global_gates;
+    *max_global = plan.max_global_operands;
+    *too_small = plan.window_too_small ? 1 : 0;
+    const std::size_t m = std::min<std::size_t>(capacity, plan.passes.size());
+    for (std::size_t i = 0; i < m; ++i) {
+        masks[i] = plan.passes[i].global_mask;
+        first[i] = static_cast<uint32_t>(plan.passes[i].first_operation);
+        end[i] = static_cast<uint32_t>(plan.passes[i].end_operation);
+    }
+    return static_cast<uint32_t>(plan.passes.size());
 }
 
 void ref_fused_block_matrix(const orc_circuit* circuit, const orc_plan* plan, uint32_t block, double* out) {

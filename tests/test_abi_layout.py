@@ -17,7 +17,8 @@ MIRRORS = {
     "mh_execution_result": H.CResult, "mh_cost_estimate": H.CCost, "mh_calibration": H.Calibration,
     "mh_batch_request": H.CBatchRequest, "mh_batch_result": H.CBatchResult,
     "mh_server_options": H.CServerOptions, "mh_job": H.CJob, "mh_job_result": H.CJobResult,
-    "mh_server_stats": H.CServerStats,
+    "mh_server_stats": H.CServerStats, "mh_tier_pass": H.CTierPass, "mh_tier_request": H.CTierRequest,
+    "mh_tier_cost": H.CTierCost,
 }
 
 
