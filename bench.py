@@ -355,6 +355,59 @@ def complex64_report(args, H, W) -> dict:
     return out
 
 
+def tiered(args, H, W) -> dict:
+    """The host-offloaded tier: the random circuit at --tier-qubits (complex128,
+    16 GiB at 30 qubits) in pinned host memory, run through two device windows
+    of 2^--tier-window amplitudes; the link bytes per second of its passes
+    against the library's measured copy rates, the in-memory engine's time for
+    the same circuit, and a full-size check (counts and terms equal to the
+    in-memory engine's)."""
+    import numpy as np
+    n, window = args.tier_qubits, args.tier_window
+    c = W.random_circuit(n, args.layers, 2025)
+    c.ops = [op for op in c.ops if op[0] != W.MEASURE]
+    cal = H.calibrate()
+    tier, dense = H.HipHostTierEngine(), H.HipDenseEngine()
+    terms = W.z_terms(n)[:4] + [((1 << (n - 1)) | 1, 1 << (n - 2)), ((1 << (n - 1)) | (1 << (n - 3)), 0)]
+    kw = dict(shots=10000, seed=7, targets=[(q, q) for q in range(n)], width=n, terms=terms)
+    est = tier.estimate(c, tier_window_qubits=window, **kw)
+    log(f"tier: {n} qubits c128, window 2^{window}, chunk 2^{est.chunk_qubits}: {est.passes} passes, "
+        f"{est.link_bytes / 2**30:.0f} GiB over the link, model {est.seconds:.1f} s")
+    r = tier.execute(c, tier_window_qubits=window, statevector=False, record_pass_times=True, **kw)
+    log(f"tier: execute {r.execute_seconds:.2f} s, passes {r.gate_seconds:.2f} s, sampling {r.sample_seconds:.2f} s, "
+        f"terms {r.expectation_seconds:.2f} s")
+    m = dense.execute(c, statevector=False, **kw)
+    log(f"in memory: execute {m.execute_seconds:.2f} s, passes {m.gate_seconds:.3f} s")
+    # The link's two directions are separate lanes: its ceiling for the passes
+    # (half their bytes each way) is the sum of the measured one-way rates.
+    link = {"h2d_GBps": round(cal.h2d_bytes_per_second / 1e9, 1), "d2h_GBps": round(cal.d2h_bytes_per_second / 1e9, 1)}
+    peak = round(link["h2d_GBps"] + link["d2h_GBps"], 1)
+    log(f"host link: {link['h2d_GBps']} GB/s in, {link['d2h_GBps']} GB/s out (pinned, one way at a time)")
+    state = (1 << n) * 16
+    passes = max(int(r.num_passes), 1)
+    pass_link = state * passes + state * (passes - 1)
+    ms = np.asarray(r.pass_ms, np.float64)
+    per_pass = (2 * state / (ms[1:] * 1e-3)).tolist() if len(ms) > 1 else []
+    return {
+        "qubits": n, "precision": "c128", "layers": args.layers, "window_qubits": int(r.cost.window_qubits),
+        "chunk_qubits": int(r.cost.chunk_qubits), "passes": int(r.num_passes),
+        "host_state_GiB": state / 2**30, "device_bytes": int(r.cost.device_bytes),
+        "link_bytes": int(r.cost.link_bytes), "model_seconds": round(float(r.cost.seconds), 3),
+        "execute_seconds": round(float(r.execute_seconds), 3), "pass_seconds": round(float(r.gate_seconds), 3),
+        "sample_seconds": round(float(r.sample_seconds), 3),
+        "expectation_seconds": round(float(r.expectation_seconds), 3),
+        "pass_link_GBps": round(pass_link / r.gate_seconds / 1e9, 1),
+        "steady_pass_link_GBps_median": round(float(np.median(per_pass)) / 1e9, 1) if per_pass else None,
+        "link_measured": link,
+        "roofline": {"bound": "host link", "achieved": round(pass_link / r.gate_seconds / 1e9, 1),
+                     "peak": peak, "unit": "GB/s", "frac": round(pass_link / r.gate_seconds / 1e9 / peak, 3),
+                     "peak_source": "sum of the measured pinned host-to-device and device-to-host rates"},
+        "in_memory_pass_seconds": round(float(m.gate_seconds), 4),
+        "counts_equal_in_memory": r.counts == m.counts,
+        "max_term_difference": float(np.max(np.abs(r.expectation - m.expectation))),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -377,7 +430,7 @@ def main() -> int:
     ap.add_argument("--group-bytes", type=int, default=0, help="batch cache-group size in bytes (0: one launch per pass)")
     ap.add_argument("--kernel-streams", type=int, default=0)
     ap.add_argument("--cached-states", action="store_true")
-    ap.add_argument("--only", choices=["batched", "served", "complex64"],
+    ap.add_argument("--only", choices=["batched", "served", "complex64", "tiered"],
                     help="run one section alone and print its JSON object (for profiling)")
     ap.add_argument("--serve-jobs", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=0.0, help="arrivals per second; 0: 0.8 x measured capacity")
@@ -388,6 +441,8 @@ def main() -> int:
     ap.add_argument("--no-c64-report", action="store_true",
                     help="skip the complex64 form at --c64-qubits and its error against complex128")
     ap.add_argument("--c64-qubits", type=int, default=34)
+    ap.add_argument("--tier-qubits", type=int, default=30)
+    ap.add_argument("--tier-window", type=int, default=28)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run the library's calibration (its streaming copy of 2 GiB each way is the "
                          "reference for scaling HBM traffic counters)")
@@ -408,7 +463,7 @@ def main() -> int:
     log(f"device {info.name.decode()} {info.arch.decode()} CUs={info.compute_units} "
         f"HBM={info.total_bytes / 2**30:.0f} GiB free={info.free_bytes / 2**30:.0f} GiB")
     if args.only:
-        section = {"batched": batched, "served": served, "complex64": complex64_report}[args.only]
+        section = {"batched": batched, "served": served, "complex64": complex64_report, "tiered": tiered}[args.only]
         print(json.dumps({args.only: section(args, H, W)}), flush=True)
         return 0
     precision = H.C128 if args.precision == "c128" else H.C64

@@ -53,6 +53,7 @@ constexpr uint32_t kMaxChosenWindowGlobals = 6;
 // The window with no device budget: 2^28 amplitudes (4 GiB at complex128).
 constexpr uint32_t kDefaultWindowQubits = 28;
 constexpr uint32_t kWindows = 2;
+constexpr uint64_t kMaxTrackedChunks = uint64_t{1} << 16;
 constexpr uint32_t kMaxQubits = 40;
 
 uint64_t state_bytes_of(uint32_t n, int32_t precision) { return (uint64_t{1} << n) * amp_bytes(precision); }
@@ -187,6 +188,29 @@ TermSets split_terms(const std::vector<std::pair<uint64_t, uint64_t>>& terms, ui
     return s;
 }
 
+// The window bit of qubit q in a pass whose (padded) set is `set`.
+uint32_t window_bit(uint32_t q, uint32_t chunk, uint64_t set) {
+    if (q < chunk) return q;
+    const uint64_t below = (uint64_t{1} << (q - chunk)) - 1;
+    return chunk + static_cast<uint32_t>(std::popcount(set & below));
+}
+
+// A pass's set padded with the lowest free global qubits up to w of them.
+uint64_t padded_set(uint64_t set, uint32_t globals, uint32_t w) {
+    for (uint32_t b = 0; b < globals && static_cast<uint32_t>(std::popcount(set)) < w; ++b) set |= uint64_t{1} << b;
+    return set;
+}
+
+// A pass's blocks with their operands remapped into its windows.
+std::vector<Block> pass_blocks(const std::vector<Block>& all_blocks, const TierPlan& tier, const TierPass& pass) {
+    const uint32_t globals = tier.n - tier.chunk;
+    const uint64_t set = padded_set(pass.mask, globals, tier.window - tier.chunk);
+    std::vector<Block> blocks(all_blocks.begin() + pass.first, all_blocks.begin() + pass.end);
+    for (Block& b : blocks)
+        for (uint32_t& q : b.operands) q = window_bit(q, tier.chunk, set);
+    return blocks;
+}
+
 struct Layout {
     Plan plan;
     TierPlan tier;
@@ -256,8 +280,7 @@ Layout lay_out(const ExecutionRequest& r, const TierRequest& t) {
         if (p > 0) link = std::max(link, sb / c.h2d_bytes_per_second);
         double compute = 0;
         if (!L.tier.passes.empty()) {
-            const TierPass& tp = L.tier.passes[p];
-            const std::vector<Block> sub(L.plan.blocks.begin() + tp.first, L.plan.blocks.begin() + tp.end);
+            const std::vector<Block> sub = pass_blocks(L.plan.blocks, L.tier, L.tier.passes[p]);
             const uint32_t wp = count_passes(window, r.precision, sub, r.window_qubits, r.max_pass_cost);
             compute = static_cast<double>(wp) * 2.0 * sb / c.pass_bytes_per_second +
                       static_cast<double>(groups * (wp + 2)) * c.launch_seconds;
@@ -320,12 +343,25 @@ public:
         synchronize();
     }
 
+    // The event after chunk c's last write-back, made on first use.
+    hipEvent_t chunk_done(uint64_t c) { return lazy(chunk_events_, c, hipEventDisableTiming); }
+    // Timing events at the start (0) and after each pass's write-backs.
+    hipEvent_t pass_end(std::size_t p) { return lazy(pass_events_, p, hipEventDefault); }
+
     hipStream_t in = nullptr, out = nullptr;
     hipEvent_t in_done[kWindows] = {}, used[kWindows] = {}, out_done[kWindows] = {};
     hipEvent_t drained = nullptr;
     bool out_pending[kWindows] = {}, use_pending[kWindows] = {};
 
 private:
+    std::vector<hipEvent_t> chunk_events_, pass_events_;
+
+    static hipEvent_t lazy(std::vector<hipEvent_t>& list, uint64_t k, unsigned flags) {
+        if (list.size() <= k) list.resize(k + 1, nullptr);
+        if (list[k] == nullptr) MH_HIP(hipEventCreateWithFlags(&list[k], flags));
+        return list[k];
+    }
+
     void release() noexcept {
         if (in) (void)hipStreamSynchronize(in);
         if (out) (void)hipStreamSynchronize(out);
@@ -338,43 +374,31 @@ private:
         }
         if (drained) (void)hipEventDestroy(drained);
         drained = nullptr;
+        for (auto* list : {&chunk_events_, &pass_events_}) {
+            for (hipEvent_t e : *list)
+                if (e) (void)hipEventDestroy(e);
+            list->clear();
+        }
         if (in) (void)hipStreamDestroy(in);
         if (out) (void)hipStreamDestroy(out);
         in = out = nullptr;
     }
 };
 
-// Copies the chunks base | deposit(s, set), s = 0 .. 2^popcount(set) - 1,
-// between the host state and window slots s; chunks adjacent in both go as one.
-void copy_group(char* host, char* window, uint64_t base, uint64_t set, uint64_t chunk_bytes, bool to_device,
-                hipStream_t stream) {
+// Calls f(first slot, first chunk, run length) for the chunks
+// base | deposit(s, set), s = 0 .. 2^popcount(set) - 1, of a group's window
+// slots, chunks adjacent in both the window and the host state as one run.
+template <typename F>
+void for_runs(uint64_t base, uint64_t set, F&& f) {
     const uint64_t slots = uint64_t{1} << std::popcount(set);
     uint64_t s = 0;
     while (s < slots) {
         const uint64_t c0 = base | deposit(s, set);
         uint64_t run = 1;
         while (s + run < slots && (base | deposit(s + run, set)) == c0 + run) ++run;
-        char* h = host + c0 * chunk_bytes;
-        char* d = window + s * chunk_bytes;
-        if (to_device)
-            MH_HIP(hipMemcpyAsync(d, h, run * chunk_bytes, hipMemcpyHostToDevice, stream));
-        else
-            MH_HIP(hipMemcpyAsync(h, d, run * chunk_bytes, hipMemcpyDeviceToHost, stream));
+        f(s, c0, run);
         s += run;
     }
-}
-
-// The window bit of qubit q in a pass whose (padded) set is `set`.
-uint32_t window_bit(uint32_t q, uint32_t chunk, uint64_t set) {
-    if (q < chunk) return q;
-    const uint64_t below = (uint64_t{1} << (q - chunk)) - 1;
-    return chunk + static_cast<uint32_t>(std::popcount(set & below));
-}
-
-// A pass's set padded with the lowest free global qubits up to w of them.
-uint64_t padded_set(uint64_t set, uint32_t globals, uint32_t w) {
-    for (uint32_t b = 0; b < globals && static_cast<uint32_t>(std::popcount(set)) < w; ++b) set |= uint64_t{1} << b;
-    return set;
 }
 
 struct TierRun {
@@ -394,36 +418,46 @@ TierRun run_tier_passes(const ExecutionRequest& r, const Layout& L, char* host, 
     std::vector<TierPass> passes = L.tier.passes;
     if (passes.empty()) passes.push_back(TierPass{0, 0, 0});
     const cplx phase = host_phase(r.global_phase);
+    const uint64_t groups = uint64_t{1} << (globals - std::min(globals, w));
+    // A load of a chunk waits for that chunk's last write-back only (an event
+    // per chunk), so a pass's first loads run beside the last write-backs of
+    // the pass before it; with more chunks than this, a pass waits for the
+    // whole of the last one instead.
+    const bool tracked = (uint64_t{1} << globals) <= kMaxTrackedChunks;
+    std::vector<uint8_t> written(tracked ? (uint64_t{1} << globals) : 0, 0);
+    const bool timing = r.record_pass_times;
+    if (timing) MH_HIP(hipEventRecord(pl.pass_end(0), streams().main));
     TierRun run;
     using clock = std::chrono::steady_clock;
     const auto t0 = clock::now();
-    auto pass_start = t0;
     for (std::size_t p = 0; p < passes.size(); ++p) {
         const uint64_t set = padded_set(passes[p].mask, globals, w);
         const uint64_t rest = all & ~set;
-        const uint64_t groups = uint64_t{1} << (globals - std::popcount(set));
-        std::vector<Block> blocks(L.plan.blocks.begin() + passes[p].first, L.plan.blocks.begin() + passes[p].end);
-        for (Block& b : blocks)
-            for (uint32_t& q : b.operands) q = window_bit(q, chunk, set);
+        const std::vector<Block> blocks = pass_blocks(L.plan.blocks, L.tier, passes[p]);
         const bool first = p == 0;
-        auto issue_in = [&](uint64_t i) {
-            const uint32_t b = static_cast<uint32_t>(i % kWindows);
+        if (!first && !tracked) MH_HIP(hipEventRecord(pl.drained, pl.out));
+        auto issue_in = [&](uint64_t g) {
+            const uint32_t b = static_cast<uint32_t>(g % kWindows);
+            char* win = static_cast<char*>(windows[b]);
             if (pl.out_pending[b]) MH_HIP(hipStreamWaitEvent(pl.in, pl.out_done[b], 0));
-            copy_group(host, static_cast<char*>(windows[b]), deposit(i, rest), set, chunk_bytes, true, pl.in);
+            if (!tracked) MH_HIP(hipStreamWaitEvent(pl.in, pl.drained, 0));
+            for_runs(deposit(g, rest), set, [&](uint64_t s0, uint64_t c0, uint64_t len) {
+                if (tracked)
+                    for (uint64_t c = c0; c < c0 + len; ++c)
+                        if (written[c]) MH_HIP(hipStreamWaitEvent(pl.in, pl.chunk_done(c), 0));
+                MH_HIP(hipMemcpyAsync(win + s0 * chunk_bytes, host + c0 * chunk_bytes, len * chunk_bytes,
+                                      hipMemcpyHostToDevice, pl.in));
+            });
             MH_HIP(hipEventRecord(pl.in_done[b], pl.in));
         };
-        if (!first) {
-            // The chunks this pass reads were written back by the last one.
-            MH_HIP(hipEventRecord(pl.drained, pl.out));
-            MH_HIP(hipStreamWaitEvent(pl.in, pl.drained, 0));
-            for (uint64_t i = 0; i < std::min<uint64_t>(groups, kWindows); ++i) issue_in(i);
-        }
-        for (uint64_t i = 0; i < groups; ++i) {
-            const uint32_t b = static_cast<uint32_t>(i % kWindows);
-            void* win = windows[b];
+        if (!first)
+            for (uint64_t g = 0; g < std::min<uint64_t>(groups, kWindows); ++g) issue_in(g);
+        for (uint64_t g = 0; g < groups; ++g) {
+            const uint32_t b = static_cast<uint32_t>(g % kWindows);
+            char* win = static_cast<char*>(windows[b]);
             if (first) {
                 if (pl.out_pending[b]) MH_HIP(hipStreamWaitEvent(streams().main, pl.out_done[b], 0));
-                if (deposit(i, rest) == 0)
+                if (deposit(g, rest) == 0)
                     launch_init(win, W, r.precision, phase.real(), phase.imag());
                 else
                     MH_HIP(hipMemsetAsync(win, 0, window_bytes, streams().main));
@@ -434,21 +468,26 @@ TierRun run_tier_passes(const ExecutionRequest& r, const Layout& L, char* host, 
                 apply_blocks(win, W, r.precision, blocks, r.window_qubits, false, nullptr, r.max_pass_cost);
             MH_HIP(hipEventRecord(pl.used[b], streams().main));
             MH_HIP(hipStreamWaitEvent(pl.out, pl.used[b], 0));
-            copy_group(host, static_cast<char*>(win), deposit(i, rest), set, chunk_bytes, false, pl.out);
+            for_runs(deposit(g, rest), set, [&](uint64_t s0, uint64_t c0, uint64_t len) {
+                MH_HIP(hipMemcpyAsync(host + c0 * chunk_bytes, win + s0 * chunk_bytes, len * chunk_bytes,
+                                      hipMemcpyDeviceToHost, pl.out));
+                if (tracked)
+                    for (uint64_t c = c0; c < c0 + len; ++c) {
+                        MH_HIP(hipEventRecord(pl.chunk_done(c), pl.out));
+                        written[c] = 1;
+                    }
+            });
             MH_HIP(hipEventRecord(pl.out_done[b], pl.out));
             pl.out_pending[b] = true;
-            if (!first && i + kWindows < groups) issue_in(i + kWindows);
+            if (!first && g + kWindows < groups) issue_in(g + kWindows);
         }
+        if (timing) MH_HIP(hipEventRecord(pl.pass_end(p + 1), pl.out));
         run.passes_run = static_cast<uint32_t>(p + 1);
-        if (r.record_pass_times || r.monitor || p + 1 == passes.size()) {
-            MH_HIP(hipStreamSynchronize(pl.out));
-            const auto now = clock::now();
-            if (r.record_pass_times)
-                run.pass_ms.push_back(static_cast<float>(std::chrono::duration<double, std::milli>(now - pass_start).count()));
-            pass_start = now;
+        if (r.monitor) {
             // The pass's safe boundary: every chunk holds the state after its
             // last block (reference chunked_executor.cpp:216-221).
-            if (r.monitor && r.monitor(r.monitor_data, p + 1) != 0) {
+            MH_HIP(hipStreamSynchronize(pl.out));
+            if (r.monitor(r.monitor_data, p + 1) != 0) {
                 pl.drain();
                 fail(MH_ERR_ABANDONED, "the execution monitor abandoned the run after " + std::to_string(p + 1) + " of " +
                                            std::to_string(passes.size()) + " passes");
@@ -457,6 +496,11 @@ TierRun run_tier_passes(const ExecutionRequest& r, const Layout& L, char* host, 
     }
     pl.drain();
     run.gate_seconds = std::chrono::duration<double>(clock::now() - t0).count();
+    if (timing) {
+        run.pass_ms.resize(passes.size());
+        for (std::size_t p = 0; p < passes.size(); ++p)
+            MH_HIP(hipEventElapsedTime(&run.pass_ms[p], pl.pass_end(p), pl.pass_end(p + 1)));
+    }
     return run;
 }
 

@@ -387,4 +387,138 @@ EngineExecutionResult HipDenseEngine::execute(const EngineExecutionRequest& requ
     return result;
 }
 
+// ---- the host-offloaded tier
+
+namespace {
+
+// The tier's refusals before anything is costed (the disk tier's,
+// disk_dense_engine.cpp:148-160, and this ABI's).
+std::optional<std::string> tier_unsupported(std::uint32_t num_qubits, std::span<const Operation> operations,
+                                            bool want_statevector, bool is_sole_region, bool want_expectation,
+                                            const Observable* observable, std::uint32_t bitstring_width) {
+    if (auto reason = unsupported(num_qubits, operations, want_expectation, observable, bitstring_width)) return reason;
+    if (want_statevector)
+        return std::string("the host-offloaded tier does not return a statevector: the state would be held twice");
+    if (!is_sole_region) return std::string("the host-offloaded tier runs only as a job's sole region");
+    return std::nullopt;
+}
+
+std::uint64_t device_budget_or_available(std::uint64_t budget) {
+    if (budget != 0) return budget;
+    std::uint64_t available = 0;
+    if (mh_device_available_bytes(&available) != MH_OK) return 0;
+    return available;
+}
+
+}  // namespace
+
+EngineCapabilities HipHostTierEngine::capabilities() const {
+    return EngineCapabilities{/*counts=*/true, /*statevector=*/false, /*expectation=*/true};
+}
+
+CostEstimate HipHostTierEngine::estimate(const EngineEstimateRequest& request) const {
+    try {
+        if (!hip_device_available()) return CostEstimate{false, 0, 0, "no HIP device is available", {}};
+        if (const auto reason = tier_unsupported(request.num_qubits, *request.operations, request.want_statevector,
+                                                 request.is_sole_region, request.want_expectation, request.observable,
+                                                 request.bitstring_width))
+            return CostEstimate{false, 0, 0, *reason, {}};
+        const FlatCircuit flat(request.num_qubits, *request.operations);
+        const Outputs o = outputs_of(request.want_expectation, request.observable, {});
+        mh_execution_request q{};
+        q.circuit = &flat.view();
+        q.precision = MH_C128;
+        q.fusion = request.fuse_gates ? MH_FUSE_CONTAINMENT : MH_FUSE_NONE;
+        q.shots = request.want_counts ? request.shots : 0;
+        q.num_terms = static_cast<std::uint32_t>(o.x.size());
+        q.x_masks = o.x.data();
+        q.z_masks = o.z.data();
+        q.budget_bytes = device_budget_or_available(device_budget_);
+        mh_tier_request t{request.tier_chunk_qubits, request.tier_window_qubits, nullptr, 0, 0};
+        mh_tier_cost c{};
+        if (mh_tier_estimate(&q, &t, &c) != MH_OK) return CostEstimate{false, 0, 0, mh_last_error(), {}};
+        if (!c.feasible) return CostEstimate{false, 0, 0, c.infeasible_reason, {}};
+        if (c.host_state_bytes > request.budget_bytes)
+            return CostEstimate{false, 0, 0,
+                                "the host-offloaded state needs " + std::to_string(c.host_state_bytes) +
+                                    " bytes of host memory, budget is " + std::to_string(request.budget_bytes),
+                                {}};
+        // Work units as the in-memory engine counts them (the same plan).
+        q.budget_bytes = 0;
+        mh_cost_estimate e{};
+        const std::uint64_t work = mh_engine_estimate(&q, &e) == MH_OK ? e.work_units : 0;
+        CostEstimate out{true, c.host_state_bytes, work, "",
+                         DeviceCost{c.device_bytes, c.link_bytes - c.link_write_bytes, c.link_write_bytes, 0}};
+        out.tier = TierCost{c.passes, c.chunk_qubits, c.window_qubits, c.link_bytes, /*file_bytes=*/0,
+                            c.link_write_bytes};
+        return out;
+    } catch (const std::exception& e) {
+        return CostEstimate{false, 0, 0, e.what(), {}};
+    }
+}
+
+EngineExecutionResult HipHostTierEngine::execute(const EngineExecutionRequest& request) const {
+    common::ensure(hip_device_available(),
+                   "HipHostTierEngine::execute: no HIP device is available -- the planner must not select this "
+                   "engine without one");
+    if (const auto reason = tier_unsupported(request.num_qubits, request.operations, request.want_statevector,
+                                             request.is_sole_region, request.want_expectation, request.observable,
+                                             request.bitstring_width))
+        throw common::UnsupportedFeature(*reason);
+    const FlatCircuit flat(request.num_qubits, request.operations);
+    const Outputs o = outputs_of(request.want_expectation, request.observable, request.measure_targets);
+    mh_execution_request q{};
+    q.circuit = &flat.view();
+    q.precision = MH_C128;
+    q.fusion = request.fuse_gates ? MH_FUSE_CONTAINMENT : MH_FUSE_NONE;
+    q.global_phase = request.global_phase;
+    q.shots = request.want_counts ? request.shots : 0;
+    q.seed = request.seed;
+    q.target_qubits = o.target_qubits.data();
+    q.target_clbits = o.target_clbits.data();
+    q.num_targets = static_cast<std::uint32_t>(o.target_qubits.size());
+    q.bitstring_width = request.bitstring_width;
+    q.num_terms = static_cast<std::uint32_t>(o.x.size());
+    q.x_masks = o.x.data();
+    q.z_masks = o.z.data();
+    q.budget_bytes = device_budget_or_available(device_budget_);
+    MonitorBridge bridge{request.monitor, nullptr};
+    if (request.monitor != nullptr) {
+        q.monitor = &MonitorBridge::call;
+        q.monitor_data = &bridge;
+    }
+    const std::uint64_t host_bytes = (std::uint64_t{1} << request.num_qubits) * sizeof(std::complex<double>);
+    if (host_bytes > request.budget_bytes)
+        throw common::ResourceExceeded("the host-offloaded state needs " + std::to_string(host_bytes) +
+                                       " bytes of host memory, budget is " + std::to_string(request.budget_bytes));
+    // The pinned state, on the host ledger while the engine holds it.
+    const common::MemoryLedger::Charge host(host_bytes);
+    mh_tier_request t{request.tier_chunk_qubits, request.tier_window_qubits, nullptr, 0, 0};
+    std::vector<std::uint64_t> outcomes(q.shots ? q.shots : 1), counts(q.shots ? q.shots : 1);
+    std::vector<double> terms(2 * o.x.size() + 2);
+    mh_execution_result res{};
+    res.outcomes = outcomes.data();
+    res.counts = counts.data();
+    res.expectation_terms = terms.data();
+    const int st = mh_tier_execute(&q, &t, &res, nullptr);
+    if (st == MH_ERR_ABANDONED && bridge.thrown) std::rethrow_exception(bridge.thrown);
+    if (st != MH_OK) throw_status(st, "HipHostTierEngine::execute");
+    { const common::MemoryLedger::Charge device(res.peak_device_bytes, common::MemoryLedger::Space::Device); }
+
+    EngineExecutionResult result;
+    result.actual_bytes = res.actual_bytes;
+    if (request.want_expectation) {
+        std::vector<std::complex<double>> raw(o.x.size());
+        for (std::size_t i = 0; i < raw.size(); ++i) raw[i] = {terms[2 * i], terms[2 * i + 1]};
+        result.expectation_terms = std::move(raw);
+    }
+    if (request.want_counts) {
+        std::map<BitString, std::uint64_t> hist;
+        for (std::uint64_t i = 0; i < res.num_outcomes; ++i)
+            hist.emplace(bitstring_of(outcomes[i], request.bitstring_width), counts[i]);
+        result.counts = std::move(hist);
+    }
+    return result;
+}
+
 }  // namespace menos::engines::dense_hip

@@ -48,6 +48,31 @@ public:
     [[nodiscard]] double device_seconds(const EngineEstimateRequest& request) const;
 };
 
+// The host-offloaded tier, (Dense, Hip, HostOffloaded): the reference's
+// Tier::HostOffloaded (include/menos/plan.h:88-98), which its planner refuses
+// as not built (src/plan/planner.cpp:386-390). The state lives in pinned host
+// memory and the passes run on the device through two windows
+// (mh_tier_estimate, mh_tier_execute) on the disk tier's pass plan. Like the
+// disk tier's engine (src/engines/dense_tiered/disk_dense_engine.h:8-40) it
+// returns no statevector, runs a job's sole region only, and takes the
+// request's tier seams (tier_chunk_qubits, tier_window_qubits); its counts and
+// raw terms are HipDenseEngine's. budget_bytes bounds the host state; the
+// windows take the device budget given here (0: the device's available bytes).
+class HipHostTierEngine final : public SimulationEngine {
+public:
+    explicit HipHostTierEngine(std::uint64_t device_budget_bytes = 0) : device_budget_(device_budget_bytes) {}
+    [[nodiscard]] std::string_view name() const override { return "dense_hip_host_offloaded"; }
+    [[nodiscard]] Engine engine_id() const override { return Engine::Dense; }
+    [[nodiscard]] Backend backend() const override { return kBackendHip; }
+    [[nodiscard]] Tier tier() const override { return Tier::HostOffloaded; }
+    [[nodiscard]] EngineCapabilities capabilities() const override;
+    [[nodiscard]] CostEstimate estimate(const EngineEstimateRequest& request) const override;
+    [[nodiscard]] EngineExecutionResult execute(const EngineExecutionRequest& request) const override;
+
+private:
+    std::uint64_t device_budget_;
+};
+
 // The operations of a region in the C ABI's flat circuit layout (the
 // reference's Circuit::append_batch layout, src/ir/circuit.cpp:34-67).
 // Holds the arrays the mh_circuit view points into.

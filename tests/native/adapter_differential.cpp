@@ -280,6 +280,8 @@ void host_cases() {
 
 // ---- device: the differential cases
 
+void tier_cases(const SimulationEngine& cpu);
+
 void device_cases() {
     const DenseEngine cpu_engine;
     const HipDenseEngine hip_engine;
@@ -434,6 +436,102 @@ void device_cases() {
         const auto engines = engines::dense_hip::with_hip_engine({&cpu_engine});
         const SimulationEngine* found = engines::dense_hip::find_engine(engines, Engine::Dense, kBackendHip);
         check(found != nullptr && found->name() == "dense_hip", "registry resolves the HIP engine");
+    }
+
+    tier_cases(cpu);
+}
+
+// The host-offloaded tier through the same interface: a device budget that
+// holds two 2^12-amplitude windows and the outputs' tables (and, once,
+// explicit seams) makes 16-18-qubit states run chunk by chunk; counts equal
+// the CPU engine's and raw terms are within 1e-12.
+void tier_cases(const SimulationEngine& cpu) {
+    const engines::dense_hip::HipHostTierEngine tier_engine(880000);
+    const SimulationEngine& tier = tier_engine;
+    check(tier.tier() == Tier::HostOffloaded && tier.backend() == kBackendHip && !tier.capabilities().statevector,
+          "tier engine identity");
+    for (std::uint64_t seed = 0; seed < 3; ++seed) {
+        const std::uint32_t n = 16 + static_cast<std::uint32_t>(seed);
+        const std::vector<Operation> ops = mixed_circuit(n, 90, 990 + seed);
+        Observable obs;
+        std::mt19937_64 g(991 + seed);
+        const std::uint64_t all = (std::uint64_t{1} << n) - 1;
+        for (int t = 0; t < 12; ++t) obs.terms.push_back({pauli(n, g() & all, g() & all), C{1, 0}});
+        std::vector<std::pair<QubitId, ClbitId>> targets;
+        for (std::uint32_t q = 0; q < n; ++q) targets.push_back({q, q});
+        for (const bool fuse : {true, false}) {
+            auto r = request(n, ops, false, true, &obs, 5000, targets, n, 0.2, fuse);
+            if (seed == 2) {
+                r.tier_chunk_qubits = 6;
+                r.tier_window_qubits = 12;
+            }
+            EngineEstimateRequest er{.num_qubits = n, .operations = &ops, .want_statevector = false,
+                                     .want_counts = true, .want_expectation = true, .observable = &obs};
+            er.budget_bytes = r.budget_bytes;
+            er.shots = 5000;
+            er.bitstring_width = n;
+            er.is_sole_region = true;
+            er.fuse_gates = fuse;
+            er.tier_chunk_qubits = r.tier_chunk_qubits;
+            er.tier_window_qubits = r.tier_window_qubits;
+            const CostEstimate e = tier.estimate(er);
+            const std::string name = "tier n=" + std::to_string(n) + (fuse ? " fused" : " unfused");
+            check(e.feasible && e.tier.passes > 1 && e.tier.window_qubits == 12u &&
+                      e.tier.link_bytes > e.tier.link_write_bytes,
+                  name + ": estimate (" + e.infeasible_reason + ")");
+            const auto a = cpu.execute(r);
+            const auto b = tier.execute(r);
+            check(a.counts && b.counts && *a.counts == *b.counts, name + ": counts");
+            double worst = 0;
+            for (std::size_t t = 0; t < obs.terms.size(); ++t)
+                worst = std::max(worst, std::abs((*a.expectation_terms)[t] - (*b.expectation_terms)[t]));
+            check(worst <= 1e-12, name + ": raw terms within 1e-12 (worst " + std::to_string(worst) + ")");
+        }
+    }
+    // Refusals: a statevector, a non-sole region, a host state over budget.
+    {
+        const std::uint32_t n = 16;
+        const std::vector<Operation> ops = mixed_circuit(n, 40, 995);
+        EngineEstimateRequest er{.num_qubits = n, .operations = &ops, .want_statevector = true,
+                                 .want_counts = false, .want_expectation = false, .observable = nullptr};
+        er.budget_bytes = std::uint64_t{1} << 40;
+        er.is_sole_region = true;
+        check(!tier.estimate(er).feasible, "tier: a statevector is infeasible");
+        er.want_statevector = false;
+        er.is_sole_region = false;
+        check(!tier.estimate(er).feasible, "tier: a non-sole region is infeasible");
+        er.is_sole_region = true;
+        er.budget_bytes = (std::uint64_t{16} << n) - 1;  // one byte short of the state
+        check(!tier.estimate(er).feasible, "tier: a host state over budget is infeasible");
+        auto r = request(n, ops, false, false, nullptr);
+        r.budget_bytes = (std::uint64_t{16} << n) - 1;
+        check(throws<common::ResourceExceeded>([&] { (void)tier.execute(r); }), "tier: execute over budget");
+    }
+    // The monitor at the tier's pass boundaries, and its exception.
+    {
+        struct Counting : ExecutionMonitor {
+            mutable std::vector<std::uint64_t> seen;
+            void at_safe_boundary(std::uint64_t crossed) const override { seen.push_back(crossed); }
+        } counting;
+        struct Abandon : ExecutionMonitor {
+            void at_safe_boundary(std::uint64_t crossed) const override {
+                if (crossed == 2) throw std::range_error("abandoned at tier pass 2");
+            }
+        } abandon;
+        const std::uint32_t n = 16;
+        const std::vector<Operation> ops = mixed_circuit(n, 80, 996);
+        Observable obs;
+        obs.terms.push_back({pauli(n, 3, 5), C{1, 0}});
+        auto r = request(n, ops, false, false, &obs);
+        r.monitor = &counting;
+        const auto watched = tier.execute(r);
+        bool ascending = counting.seen.size() > 2 && counting.seen.front() == 1;
+        for (std::size_t i = 1; i < counting.seen.size(); ++i) ascending = ascending && counting.seen[i] == i + 1;
+        const auto a = cpu.execute(request(n, ops, false, false, &obs));
+        check(ascending && std::abs((*a.expectation_terms)[0] - (*watched.expectation_terms)[0]) <= 1e-12,
+              "tier: monitor boundaries, same terms");
+        r.monitor = &abandon;
+        check(throws<std::range_error>([&] { (void)tier.execute(r); }), "tier: a throwing monitor abandons the run");
     }
 }
 

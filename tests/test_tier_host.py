@@ -201,3 +201,28 @@ def test_refusals():
     assert not bad.feasible and b"window_qubits" in bad.infeasible_reason
     assert not ENGINE.estimate(c, tier_chunk_qubits=13, tier_window_qubits=12).feasible
     assert not ENGINE.estimate(c, terms=[(1 << 14, 0)]).feasible
+
+
+def test_time_model_under_a_synthetic_profile():
+    """With the window passes priced at nothing, a pass takes its copies: the
+    first only writes the state back, later ones read it and write it back at
+    once (the two directions overlap), and each output read takes one state's
+    copy in."""
+    old = H.get_calibration()
+    profile = dict(hbm_bytes_per_second=6.0e12, pass_bytes_per_second=1.0e30, launch_seconds=0.0,
+                   h2d_bytes_per_second=5.0e10, d2h_bytes_per_second=4.0e10, fp64_ops_per_second=3.0e13,
+                   mfma_flops_per_second=1.0e14)
+    H.set_calibration(H.Calibration(**profile))
+    try:
+        c = _random(22, 3, 4)
+        sb = 16 << 22
+        for shots, terms, reads in ((0, (), 0), (100, [(1, 2)], 1), (100, [(1, 2), ((1 << 21) | 1, 0)], 2)):
+            e = ENGINE.estimate(c, tier_window_qubits=16, shots=shots, terms=terms)
+            p = e.passes
+            want = sb / 4.0e10 + (p - 1) * max(sb / 5.0e10, sb / 4.0e10) + reads * sb / 5.0e10
+            assert e.seconds == pytest.approx(want, rel=1e-12)
+        # Priced window passes only add time.
+        H.set_calibration(H.Calibration(**dict(profile, pass_bytes_per_second=1.0e11, launch_seconds=1e-5)))
+        assert ENGINE.estimate(c, tier_window_qubits=16).seconds > want - reads * sb / 5.0e10
+    finally:
+        H.set_calibration(old)
