@@ -442,6 +442,7 @@ def main() -> int:
                     help="skip the complex64 form at --c64-qubits and its error against complex128")
     ap.add_argument("--c64-qubits", type=int, default=34)
     ap.add_argument("--tier-qubits", type=int, default=30)
+    ap.add_argument("--no-tier", action="store_true", help="skip the host-offloaded tier section (about 25 s)")
     ap.add_argument("--tier-window", type=int, default=28)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run the library's calibration (its streaming copy of 2 GiB each way is the "
@@ -540,6 +541,8 @@ def main() -> int:
         out["served"] = served(args, H, W)
     if not args.no_c64_report:
         out["complex64"] = complex64_report(args, H, W)
+    if not args.no_tier:
+        out["tiered"] = tiered(args, H, W)
     if not args.no_cpu:
         threads = host_cores()
         sample = W.random_circuit(args.cpu_qubits, args.cpu_layers, 2025)

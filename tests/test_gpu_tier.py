@@ -134,3 +134,33 @@ def test_the_monitor_abandons_at_a_pass_boundary(hip):
     got = TIER.execute(c, tier_chunk_qubits=10, tier_window_qubits=12, monitor=seen.append)
     assert seen == list(range(1, cost.passes + 1))
     assert np.array_equal(_bits(got.statevector), _bits(DENSE.execute(c).statevector))
+
+
+@pytest.mark.parametrize("name", ["uniform", "one_window", "sparse_windows"])
+def test_counts_across_windows_of_adversarial_states(hip, name):
+    """States whose running sums stress the carried window boundaries: the
+    uniform superposition (every probability equal, exact ties at every
+    boundary), all weight in one window of 32 (the others sum to exactly
+    zero), and weight in a run of eight windows in the upper half only."""
+    n, window = 16, 11
+    c = W.Circuit(n)
+    if name == "uniform":
+        for q in range(n):
+            c.append(W.H, (q,))
+    elif name == "one_window":
+        for q in range(window):
+            c.append(W.H, (q,))
+        c.append(W.X, (n - 2,))
+    else:
+        for q in range(window):
+            c.append(W.H, (q,))
+        for q in range(window, n - 2):
+            c.append(W.H, (q,))
+        c.append(W.X, (n - 1,))
+    targets = [(q, q) for q in range(n)]
+    for seed in (3, 4):
+        kw = dict(shots=30000, seed=seed, targets=targets, width=n)
+        got = TIER.execute(c, tier_window_qubits=window, statevector=False, **kw)
+        want = DENSE.execute(c, statevector=False, **kw)
+        assert got.counts == want.counts
+
