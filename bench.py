@@ -355,6 +355,35 @@ def complex64_report(args, H, W) -> dict:
     return out
 
 
+def sampler_report(args, H, W) -> dict:
+    """Exact sampling of the 16-layer random circuit's final state at --qubits
+    (complex128): the sampler's device time and its HBM rate. Its tables read
+    the state twice (block sums, then the block transfers); each shot then
+    walks one block of 2^13 amplitudes."""
+    n = args.qubits
+    c = W.random_circuit(n, args.layers, 2025)
+    c.ops = [op for op in c.ops if op[0] != W.MEASURE]
+    eng = H.HipDenseEngine()
+    out = {"qubits": n, "precision": "c128", "shots": args.sampler_shots}
+    targets = [(q, q) for q in range(n)]
+    runs = []
+    for i in range(2):
+        r = eng.execute(c, shots=args.sampler_shots, seed=7 + i, targets=targets, width=n, statevector=False)
+        runs.append(float(r.sample_seconds))
+        log(f"sampler {n} qubits, {args.sampler_shots} shots: {r.sample_seconds * 1e3:.1f} ms "
+            f"({len(r.counts)} distinct outcomes)")
+    state = (1 << n) * 16
+    best = min(runs)
+    table_bytes = 2 * state
+    out.update({"sample_seconds": [round(x, 4) for x in runs],
+                "state_reads_GBps": round(table_bytes / best / 1e9, 1),
+                "roofline": {"bound": "hbm", "achieved": round(table_bytes / best / 1e9, 1), "peak": PEAK_HBM_GBS,
+                             "unit": "GB/s", "frac": round(table_bytes / best / 1e9 / PEAK_HBM_GBS, 4),
+                             "algorithmic_bytes": table_bytes,
+                             "note": "two reads of the state over the whole sampling call (host wall time)"}})
+    return out
+
+
 def tiered(args, H, W) -> dict:
     """The host-offloaded tier: the random circuit at --tier-qubits (complex128,
     16 GiB at 30 qubits) in pinned host memory, run through two device windows
@@ -430,7 +459,7 @@ def main() -> int:
     ap.add_argument("--group-bytes", type=int, default=0, help="batch cache-group size in bytes (0: one launch per pass)")
     ap.add_argument("--kernel-streams", type=int, default=0)
     ap.add_argument("--cached-states", action="store_true")
-    ap.add_argument("--only", choices=["batched", "served", "complex64", "tiered"],
+    ap.add_argument("--only", choices=["batched", "served", "complex64", "tiered", "sampler"],
                     help="run one section alone and print its JSON object (for profiling)")
     ap.add_argument("--serve-jobs", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=0.0, help="arrivals per second; 0: 0.8 x measured capacity")
@@ -441,6 +470,7 @@ def main() -> int:
     ap.add_argument("--no-c64-report", action="store_true",
                     help="skip the complex64 form at --c64-qubits and its error against complex128")
     ap.add_argument("--c64-qubits", type=int, default=34)
+    ap.add_argument("--sampler-shots", type=int, default=10000)
     ap.add_argument("--tier-qubits", type=int, default=30)
     ap.add_argument("--no-tier", action="store_true", help="skip the host-offloaded tier section (about 25 s)")
     ap.add_argument("--tier-window", type=int, default=28)
@@ -464,7 +494,8 @@ def main() -> int:
     log(f"device {info.name.decode()} {info.arch.decode()} CUs={info.compute_units} "
         f"HBM={info.total_bytes / 2**30:.0f} GiB free={info.free_bytes / 2**30:.0f} GiB")
     if args.only:
-        section = {"batched": batched, "served": served, "complex64": complex64_report, "tiered": tiered}[args.only]
+        section = {"batched": batched, "served": served, "complex64": complex64_report, "tiered": tiered,
+                   "sampler": sampler_report}[args.only]
         print(json.dumps({args.only: section(args, H, W)}), flush=True)
         return 0
     precision = H.C128 if args.precision == "c128" else H.C64
