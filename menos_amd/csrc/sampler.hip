@@ -188,30 +188,42 @@ __global__ void __launch_bounds__(1024) approx_prefix_kernel(SampleArgs a) {
     if (threadIdx.x == 1023) a.approx[a.blocks] = run;
 }
 
+// Amplitudes of a block staged in LDS at a time by block_xfer_kernel: 16 KiB,
+// so that several workgroups share a CU (a whole block of 2^13, 64 KiB, left
+// room for two and the kernel latency-bound).
+constexpr uint32_t kXferStage = 2048;
+
 template <typename V>
 __global__ void __launch_bounds__(kThreads) block_xfer_kernel(SampleArgs a) {
-    __shared__ double p_sh[1u << kBlockLog];
+    __shared__ double p_sh[kXferStage];
     __shared__ Xfer red[kThreads];
     const V* st = static_cast<const V*>(a.state);
     const uint32_t b = blockIdx.x;
     const int32_t s = scale_of(a.approx[b]);
     const uint64_t i0 = uint64_t(b) * a.block;
-    for (uint32_t i = threadIdx.x; i < a.block; i += kThreads) p_sh[i] = prob(st, i0 + i);
-    __syncthreads();
-    // Each thread composes a contiguous run in order; then a fixed tree (the
-    // composition is exact, so the order only has to respect the sequence).
-    const uint32_t per = static_cast<uint32_t>((a.block + kThreads - 1) / kThreads);
-    const uint32_t j0 = min(static_cast<uint32_t>(a.block), threadIdx.x * per);
-    const uint32_t j1 = min(static_cast<uint32_t>(a.block), j0 + per);
-    Xfer acc = identity_xfer(s);
-    for (uint32_t j = j0; j < j1; ++j) acc = compose(acc, element_xfer(p_sh[j], s));
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int stride = 1; stride < kThreads; stride <<= 1) {
-        if ((threadIdx.x % (2 * stride)) == 0) red[threadIdx.x] = compose(red[threadIdx.x], red[threadIdx.x + stride]);
+    const uint32_t stage = static_cast<uint32_t>(min(a.block, uint64_t{kXferStage}));
+    Xfer total = identity_xfer(s);  // thread 0's: the stages composed in order
+    for (uint64_t base = 0; base < a.block; base += stage) {
+        for (uint32_t i = threadIdx.x; i < stage; i += kThreads) p_sh[i] = prob(st, i0 + base + i);
         __syncthreads();
+        // Each thread composes a contiguous run in order; then a fixed tree
+        // (the composition is exact, so the order only has to respect the
+        // sequence). The tree's barriers also end every read of p_sh and red
+        // before the next stage writes them.
+        const uint32_t per = (stage + kThreads - 1) / kThreads;
+        const uint32_t j0 = min(stage, threadIdx.x * per);
+        const uint32_t j1 = min(stage, j0 + per);
+        Xfer acc = identity_xfer(s);
+        for (uint32_t j = j0; j < j1; ++j) acc = compose(acc, element_xfer(p_sh[j], s));
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int stride = 1; stride < kThreads; stride <<= 1) {
+            if ((threadIdx.x % (2 * stride)) == 0) red[threadIdx.x] = compose(red[threadIdx.x], red[threadIdx.x + stride]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) total = compose(total, red[0]);
     }
-    if (threadIdx.x == 0) a.xfer[b] = red[0];
+    if (threadIdx.x == 0) a.xfer[b] = total;
 }
 
 __global__ void __launch_bounds__(kThreads) group_xfer_kernel(SampleArgs a) {
