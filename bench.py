@@ -567,9 +567,9 @@ def main() -> int:
                      "algorithmic_bytes_per_launch": pass_bytes},
     }
     if not args.no_tier:
-        # Before the other sections: late in a long process the same run has
-        # measured a third slower over the host link (pinned pages allocated
-        # after many others).
+        # Before the sections that free device memory: for some tens of seconds
+        # after a large free (the served section's arena) the host link is
+        # shared, and the same run measured a third slower.
         out["tiered"] = tiered(args, H, W)
     if not args.no_batch:
         out["batched"] = batched(args, H, W)
