@@ -402,7 +402,13 @@ def tiered(args, H, W) -> dict:
     est = tier.estimate(c, tier_window_qubits=window, **kw)
     log(f"tier: {n} qubits c128, window 2^{window}, chunk 2^{est.chunk_qubits}: {est.passes} passes, "
         f"{est.link_bytes / 2**30:.0f} GiB over the link, model {est.seconds:.1f} s")
-    r = tier.execute(c, tier_window_qubits=window, statevector=False, record_pass_times=True, **kw)
+    monitor = None
+    if args.tier_progress:
+        # A line per pass (the monitor waits for each pass's write-backs, so
+        # passes no longer overlap at their boundaries: for long runs only).
+        def monitor(done):
+            log(f"tier pass {done}/{est.passes}")
+    r = tier.execute(c, tier_window_qubits=window, statevector=False, record_pass_times=True, monitor=monitor, **kw)
     log(f"tier: execute {r.execute_seconds:.2f} s, passes {r.gate_seconds:.2f} s, sampling {r.sample_seconds:.2f} s, "
         f"terms {r.expectation_seconds:.2f} s")
     m = dense.execute(c, statevector=False, **kw)
@@ -473,6 +479,8 @@ def main() -> int:
     ap.add_argument("--sampler-shots", type=int, default=10000)
     ap.add_argument("--tier-qubits", type=int, default=30)
     ap.add_argument("--no-tier", action="store_true", help="skip the host-offloaded tier section (about 25 s)")
+    ap.add_argument("--tier-progress", action="store_true",
+                    help="log each tier pass (waits at every pass boundary; for runs of minutes)")
     ap.add_argument("--tier-window", type=int, default=28)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run the library's calibration (its streaming copy of 2 GiB each way is the "
