@@ -76,11 +76,23 @@ def test_wide_blocks_across_global_qubits(hip):
         assert np.array_equal(_bits(got.statevector), _bits(want.statevector))
 
 
+def test_autotuned_fusion(hip):
+    # The autotuner's plan (containment or a width of 2-6, priced by the time
+    # model) is the plan the tier runs, the same one the in-memory engine picks.
+    c = _random(16, 5, 12)
+    got = TIER.execute(c, tier_chunk_qubits=10, tier_window_qubits=13, fusion=H.FUSE_AUTO)
+    want = DENSE.execute(c, fusion=H.FUSE_AUTO)
+    assert (got.fusion_used, got.fusion_width_used) == (want.fusion_used, want.fusion_width_used)
+    assert np.array_equal(_bits(got.statevector), _bits(want.statevector))
+
+
 def test_complex64(hip, orc):
     c = _random(18, 4, 11)
     terms = _terms(18, 13, 11)
-    got = TIER.execute(c, tier_window_qubits=13, precision=H.C64, terms=terms)
-    want = DENSE.execute(c, precision=H.C64, terms=terms)
+    kw = dict(precision=H.C64, terms=terms, shots=20000, seed=5, targets=[(q, q) for q in range(18)], width=18)
+    got = TIER.execute(c, tier_window_qubits=13, **kw)
+    want = DENSE.execute(c, **kw)
+    assert got.counts == want.counts
     sv, _, ex = orc.execute(c, terms=terms)
     assert np.max(np.abs(got.statevector.astype(np.complex128) - sv)) <= 1e-5
     assert np.max(np.abs(got.statevector - want.statevector)) <= 1e-5
