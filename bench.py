@@ -229,6 +229,7 @@ def served(args, H, W) -> dict:
                        f"{args.serve_estimator_precision}), 30% sampler (U3 + CZ brick, 20 layers, 1000 shots, "
                        "c128); 40% parameter-only updates; 256 GB ledger budget; complex64 jobs with pair-keeping "
                        "fusion",
+           "large_job_bytes": args.serve_large_bytes,
            "jobs": len(jobs),
            "estimators": sum(s.kind == "estimator" for s in specs),
            "updates": sum(s.update for s in specs)}
@@ -236,7 +237,8 @@ def served(args, H, W) -> dict:
         if not args.serve_rate:
             raise SystemExit("--serve-skip-capacity needs --serve-rate")
         return served_poisson(args, H, jobs_of, gaps, out, args.serve_rate)
-    with H.Server(budget_bytes=256_000_000_000, reserve_bytes=256_000_000_000, complex64_pairs=True) as srv:
+    with H.Server(budget_bytes=256_000_000_000, reserve_bytes=256_000_000_000, complex64_pairs=True,
+                  large_job_bytes=args.serve_large_bytes) as srv:
         w0 = time.perf_counter()
         res = srv.replay(jobs, np.zeros(len(jobs)))
         wall = time.perf_counter() - w0
@@ -297,7 +299,8 @@ def served_poisson(args, H, jobs_of, gaps, out, rate) -> dict:
     import numpy as np
     arrivals = gaps / rate
     jobs = jobs_of()
-    with H.Server(budget_bytes=256_000_000_000, reserve_bytes=256_000_000_000, complex64_pairs=True) as srv:
+    with H.Server(budget_bytes=256_000_000_000, reserve_bytes=256_000_000_000, complex64_pairs=True,
+                  large_job_bytes=args.serve_large_bytes) as srv:
         w0 = time.perf_counter()
         res = srv.replay(jobs, arrivals) if len(jobs) <= 2000 else replay_with_progress(srv, jobs, arrivals)
         wall = time.perf_counter() - w0
@@ -470,6 +473,9 @@ def main() -> int:
     ap.add_argument("--serve-jobs", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=0.0, help="arrivals per second; 0: 0.8 x measured capacity")
     ap.add_argument("--serve-estimator-precision", choices=["c64", "c128"], default="c64")
+    ap.add_argument("--serve-large-bytes", type=int, default=1 << 32,
+                    help="server large_job_bytes: jobs with states of this many bytes or more (4 GiB: 28 qubits "
+                         "complex128, 29 complex64) run one at a time in rounds of their own; 0: arrival order only")
     ap.add_argument("--serve-skip-capacity", action="store_true",
                     help="only the Poisson replay, at --serve-rate (for the 20,000-job configuration)")
     ap.add_argument("--no-serve", action="store_true")

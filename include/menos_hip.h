@@ -553,6 +553,11 @@ typedef struct mh_server_options {
                                        budget); 0: the arena grows with the rounds */
     int32_t complex64_pairs;        /* nonzero: complex64 jobs use MH_FUSE_PAIRS (complex128 jobs keep `fusion`) */
     uint32_t lanes;                 /* rounds that may run at once; 0: 3, at most 8 */
+    uint32_t reserved0;
+    uint64_t large_job_bytes;       /* nonzero: a job whose state has at least these bytes runs in a round of its
+                                       own, and while one such round runs the others wait in arrival order as
+                                       smaller jobs pass them (two large states sharing the device both run
+                                       about twice as long); 0: arrival order only */
 } mh_server_options;
 
 typedef enum mh_job_kind { MH_JOB_ESTIMATOR = 0, MH_JOB_SAMPLER = 1 } mh_job_kind;

@@ -414,21 +414,35 @@ void Server::add_segment_locked(uint64_t bytes) {
     arena_bytes_ += bytes;
 }
 
-// Admits the longest arrival-order prefix of the queue whose states fit both
-// the budget the other lanes leave and one free range of the arena. When no
+// Admits the longest arrival-order run of the queue whose states fit both
+// the budget the other lanes leave and one free range of the arena. With
+// large_job_bytes set, a large job is admitted only at the head of a round,
+// as a round of its own, and only while no other large round runs; until
+// then it keeps its place and the smaller jobs behind it may pass. When no
 // range fits, a segment is added (at least as large as the arena so far, up to
 // the budget in all), after releasing segments no round holds if the budget
 // requires it.
 bool Server::admit_locked(Batch& b) {
     std::vector<uint64_t> sizes;
+    std::vector<std::size_t> picked;  // queue positions, ascending
     uint64_t total = 0;
     const uint64_t room = budget_ - std::min(budget_, held_);
-    for (const JobPtr& j : queue_) {
-        const uint64_t s = backend_->state_bytes(j->circuit.view.num_qubits, j->precision);
+    const uint64_t large = options_.large_job_bytes;
+    for (std::size_t i = 0; i < queue_.size(); ++i) {
+        const Job& j = *queue_[i];
+        const uint64_t s = backend_->state_bytes(j.circuit.view.num_qubits, j.precision);
+        const bool is_large = large != 0 && s >= large;
+        if (is_large && large_running_ > 0) continue;
+        if (is_large && !picked.empty()) break;
         if (s > budget_ || total + s > room) break;
         if (options_.max_round_jobs && sizes.size() >= options_.max_round_jobs) break;
         sizes.push_back(s);
+        picked.push_back(i);
         total += s;
+        if (is_large) {
+            b.large = true;
+            break;
+        }
     }
     while (!sizes.empty()) {
         unsigned char* segment = nullptr;
@@ -460,12 +474,13 @@ bool Server::admit_locked(Batch& b) {
         }
         total -= sizes.back();
         sizes.pop_back();
+        picked.pop_back();
+        if (picked.empty()) b.large = false;
     }
     if (sizes.empty()) return false;
-    for (std::size_t k = 0; k < sizes.size(); ++k) {
-        b.jobs.push_back(std::move(queue_.front()));
-        queue_.pop_front();
-    }
+    for (std::size_t k : picked) b.jobs.push_back(std::move(queue_[k]));
+    for (std::size_t k = picked.size(); k-- > 0;) queue_.erase(queue_.begin() + static_cast<std::ptrdiff_t>(picked[k]));
+    if (b.large) ++large_running_;
     held_ += b.bytes;
     peak_held_ = std::max(peak_held_, held_);
     peak_round_bytes_ = std::max(peak_round_bytes_, b.bytes);
@@ -531,6 +546,7 @@ void Server::lane_loop(uint32_t lane) {
         const double r1 = now();
         lock.lock();
         --running_;
+        if (b.large) --large_running_;
         busy_seconds_ += r1 - r0;
         held_ -= b.bytes;
         free_range_locked(b.segment, b.offset, b.bytes);

@@ -171,6 +171,7 @@ private:
         uint64_t offset = 0;                // in the segment
         unsigned char* base = nullptr;      // segment + offset
         uint32_t number = 0;
+        bool large = false;                 // holds a job of at least large_job_bytes
     };
     // A segment of the arena and its free ranges (offset -> bytes).
     struct Segment {
@@ -195,6 +196,7 @@ private:
     std::chrono::steady_clock::time_point t0_;
     uint64_t budget_ = 0;
     uint32_t threads_ = 1;
+    uint32_t large_running_ = 0;  // rounds holding a large job that are running
     uint32_t lanes_ = 1;
 
     std::mutex mu_;

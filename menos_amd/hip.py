@@ -656,7 +656,8 @@ class CServerOptions(ctypes.Structure):
     _fields_ = [("budget_bytes", ctypes.c_uint64), ("max_round_jobs", ctypes.c_uint32),
                 ("host_threads", ctypes.c_uint32), ("fusion", ctypes.c_int32), ("fusion_width", ctypes.c_uint32),
                 ("plan_cache_capacity", ctypes.c_uint32), ("reserve_bytes", ctypes.c_uint64),
-                ("complex64_pairs", ctypes.c_int32), ("lanes", ctypes.c_uint32)]
+                ("complex64_pairs", ctypes.c_int32), ("lanes", ctypes.c_uint32), ("reserved0", ctypes.c_uint32),
+                ("large_job_bytes", ctypes.c_uint64)]
 
 
 class CJob(ctypes.Structure):
@@ -756,9 +757,10 @@ class Server:
     """A resident job server (mh_server_*): submit jobs, wait for results."""
 
     def __init__(self, *, budget_bytes=0, max_round_jobs=0, host_threads=0, fusion=FUSE_CONTAINMENT,
-                 fusion_width=0, plan_cache_capacity=0, reserve_bytes=0, complex64_pairs=False, lanes=0):
+                 fusion_width=0, plan_cache_capacity=0, reserve_bytes=0, complex64_pairs=False, lanes=0,
+                 large_job_bytes=0):
         o = CServerOptions(budget_bytes, max_round_jobs, host_threads, fusion, fusion_width, plan_cache_capacity,
-                           reserve_bytes, int(bool(complex64_pairs)), lanes)
+                           reserve_bytes, int(bool(complex64_pairs)), lanes, 0, large_job_bytes)
         self._h = ctypes.c_void_p()
         check(lib().mh_server_create(ctypes.byref(o), ctypes.byref(self._h)))
         self._pending = {}

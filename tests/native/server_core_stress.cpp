@@ -8,6 +8,7 @@
 // and waiters; arrival replay; admission against the budget and overlapping
 // rounds; failures at submission, planning and execution; shutdown with jobs
 // nobody waited for.
+#include <algorithm>
 #include <atomic>
 #include <cassert>
 #include <chrono>
@@ -289,6 +290,42 @@ void overlap_and_admission() {
     ok.check_result();
 }
 
+void large_jobs_one_at_a_time() {
+    // With large_job_bytes set, large jobs run in rounds of their own, one at
+    // a time, while the small jobs behind them pass; every result is still
+    // the job's own, and nothing is left in the queue.
+    mh_server_options o = options(1ull << 30, 3);
+    o.large_job_bytes = uint64_t{16} << 16;   // complex128 states of 16+ qubits
+    Server s(o, std::make_unique<StubBackend>(1ull << 32));
+    std::vector<std::unique_ptr<TestJob>> jobs;
+    std::vector<uint64_t> ts;
+    for (int i = 0; i < 12; ++i) {
+        const bool large = i % 3 == 0;
+        jobs.push_back(std::make_unique<TestJob>(large ? 18 : 6, 0.1 * i, large ? MH_JOB_SAMPLER : MH_JOB_ESTIMATOR,
+                                                 MH_C128));
+        ts.push_back(s.submit(&jobs.back()->job));
+    }
+    for (std::size_t i = 0; i < jobs.size(); ++i) {
+        s.wait(ts[i], &jobs[i]->result);
+        jobs[i]->check_result();
+    }
+    // Large rounds never overlapped: their [started, completed] intervals are
+    // disjoint; each ran in a round of its own.
+    std::vector<std::pair<double, double>> spans;
+    for (std::size_t i = 0; i < jobs.size(); i += 3)
+        spans.push_back({jobs[i]->result.started_seconds, jobs[i]->result.completed_seconds});
+    std::sort(spans.begin(), spans.end());
+    for (std::size_t k = 1; k < spans.size(); ++k) CHECK(spans[k].first >= spans[k - 1].second);
+    for (std::size_t i = 0; i < jobs.size(); i += 3)
+        for (std::size_t k = 0; k < jobs.size(); ++k)
+            if (k != i) CHECK(jobs[k]->result.round != jobs[i]->result.round);
+    // The small jobs did not wait for the large ones behind them.
+    CHECK(jobs[1]->result.completed_seconds < jobs[9]->result.completed_seconds);
+    mh_server_stats st{};
+    s.stats(&st);
+    CHECK(st.jobs_completed == jobs.size() && st.jobs_failed == 0);
+}
+
 void failures_and_tickets() {
     Server s(options(1ull << 30, 2), std::make_unique<StubBackend>(1ull << 32));
     TestJob few(2, 0.5, MH_JOB_ESTIMATOR);            // below min_qubits
@@ -383,6 +420,7 @@ int main() {
     closed_loop_and_early_completion();
     concurrent_submitters_and_waiters();
     overlap_and_admission();
+    large_jobs_one_at_a_time();
     failures_and_tickets();
     replay();
     arena_growth_and_fragmentation();
