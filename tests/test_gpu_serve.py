@@ -69,6 +69,38 @@ def test_server_matches_engine_and_oracle(orc, hip):
         assert st.jobs_completed == len(jobs) and st.jobs_failed == 0
 
 
+def test_large_jobs_one_at_a_time(hip):
+    """large_job_bytes: jobs whose states reach it run in rounds of their own
+    and never at once, smaller ones pass them, and every result is still the
+    single engine's."""
+    large = 16 << 16  # complex128 states of 16+ qubits
+    jobs = []
+    with hip.Server(large_job_bytes=large, lanes=3) as s:
+        for i in range(9):
+            if i % 3 == 0:
+                c, j = _sampler(hip, 16 + i % 2, 4, 80 + i)
+                jobs.append(("s", c, None, j, s.submit(j)))
+            else:
+                c, obs, j = _estimator(hip, 10 + i % 3, 2, 90 + i)
+                jobs.append(("e", c, obs, j, s.submit(j)))
+        results = [s.wait(t) for *_, t in jobs]
+        st = s.stats()
+    assert st.jobs_completed == len(jobs) and st.jobs_failed == 0
+    big = [r for (kind, *_), r in zip(jobs, results) if kind == "s"]
+    spans = sorted((r.started, r.completed) for r in big)
+    assert all(b[0] >= a[1] for a, b in zip(spans, spans[1:]))
+    big_rounds = {r.round for r in big}
+    assert len(big_rounds) == len(big)
+    assert not any(r.round in big_rounds for (kind, *_), r in zip(jobs, results) if kind == "e")
+    for (kind, c, obs, j, _), r in zip(jobs, results):
+        if kind == "e":
+            _check_estimator(hip, c, obs, r)
+        else:
+            single = hip.HipDenseEngine().execute(c, shots=j.shots, seed=j.c.seed, targets=c.measure_targets(),
+                                                  width=c.num_qubits, statevector=False)
+            assert r.counts == single.counts
+
+
 def test_parameter_updates_reuse_the_plan(hip):
     c, obs, j = _estimator(hip, 12, 3, 5)
     c2, _, _ = _estimator(hip, 12, 3, 6)  # same structure, fresh angles
