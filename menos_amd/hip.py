@@ -754,7 +754,9 @@ class Job:
 
 
 class Server:
-    """A resident job server (mh_server_*): submit jobs, wait for results."""
+    """A resident job server (mh_server_*): submit jobs, wait for results.
+    ``large_job_bytes``: jobs whose states reach it run one at a time in rounds
+    of their own while smaller jobs pass them (0: arrival order only)."""
 
     def __init__(self, *, budget_bytes=0, max_round_jobs=0, host_threads=0, fusion=FUSE_CONTAINMENT,
                  fusion_width=0, plan_cache_capacity=0, reserve_bytes=0, complex64_pairs=False, lanes=0,
